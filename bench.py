@@ -1,0 +1,175 @@
+"""Benchmark of the phase-field LBM timestep on one MI355X.
+
+Workload (one "step" = one full timestep of the whole grid): round jet,
+D3Q27 regularized hydrodynamics + D3Q7 phase field, 256x256x1024 cells,
+nozzle D=32, u=0.05, Re=5000, We=500, FP32 populations.  Inputs are
+resident in HBM before timing starts.  Also reported: the FP16-storage form
+of the same workload, the Laplace pressure error of static droplets
+(R = 8, 12, 16) and the CPU oracle timed on the host on a bounded sample.
+
+MLUPS counts nx*ny*nz cells per step, as the reference does
+(src/main.cu:203-205); the interior-cell rate is reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_bytes_per_cell(q, s):
+    """Algorithmic bytes of the fused collide-stream kernel per interior cell:
+    read and write Q f populations, write 7 g populations, read phi."""
+    return 2 * q * s + 7 * s + 4
+
+
+def run_form(pf, cfg, steps, warmup):
+    sim = pf.Simulation(cfg)
+    sim.init()
+    sim.step(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    tm = sim.time_steps(steps)  # HIP events on the simulation's stream
+    sim.step(1, save=True)
+    d = sim.diagnostics()
+    sim.close()
+    return wall, tm, d
+
+
+def laplace(pf, presets, R, steps=3000):
+    cfg = presets.static_droplet(n=64, radius=float(R), sigma=0.01)
+    with pf.Simulation(cfg) as sim:
+        sim.init().step(steps, save=True)
+        rho = sim.field("rho").astype(np.float64)
+        ux, uy, uz = (sim.field(k).astype(np.float64) for k in ("ux", "uy", "uz"))
+    n = cfg.nx
+    z, y, x = np.mgrid[0:n, 0:n, 0:n]
+    c = (n - 1) / 2
+    r = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    interior = (x > 0) & (x < n - 1) & (y > 0) & (y < n - 1) & (z > 0) & (z < n - 1)
+    inside = rho[(r < R - 4)].mean()
+    outside = rho[(r > R + 4) & interior].mean()
+    dp = (inside - outside) / 3.0
+    want = 2 * cfg.sigma / R
+    umax = float(np.sqrt(ux ** 2 + uy ** 2 + uz ** 2)[interior].max())
+    return {"R": R, "dp": dp, "dp_exact": want, "rel_err": abs(dp - want) / want, "spurious_umax": umax}
+
+
+def cpu_baseline(cfg, steps):
+    import oracle
+    nthr = oracle.num_threads()
+    o = oracle.Oracle(cfg, 32).init()
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    cells = cfg.nx * cfg.ny * cfg.nz
+    o.close()
+    return {"value": cells * steps / dt / 1e6, "unit": "MLUPS", "cores": nthr, "kind": "port",
+            "sample": f"C++/OpenMP FP32 oracle, same {cfg.nx}x{cfg.ny}x{cfg.nz} D3Q{cfg.velocity_set} jet, "
+                      f"{steps} timesteps after initialisation (wall clock of the steps only)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip FP16 form and Laplace probes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the timestep "
+              "would shard as z-slabs across GPUs, which this build does not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import phasefieldlbm_amd as pf
+    from phasefieldlbm_amd import presets
+
+    cfg = presets.jet_d3q27_256x256x1024()
+    cells = cfg.nx * cfg.ny * cfg.nz
+    inner = (cfg.nx - 2) * (cfg.ny - 2) * (cfg.nz - 2)
+    log(f"[bench] workload D3Q27 jet {cfg.nx}x{cfg.ny}x{cfg.nz} FP32, warmup {args.warmup}, steps {args.steps}")
+    wall, tm, diag = run_form(pf, cfg, args.steps, args.warmup)
+    ms = wall * 1e3 / args.steps
+    mlups = cells * args.steps / wall / 1e6
+    b_alg = presets.bytes_per_update(cfg)
+    step_ms = tm["step_ms"] / args.steps
+    kbytes = kernel_bytes_per_cell(cfg.velocity_set, 4) * inner
+    achieved = kbytes / (step_ms * 1e-3) / 1e9
+    log(f"[bench] fp32: {mlups:.0f} MLUPS, {ms:.3f} ms/step, kernel {step_ms:.3f} ms, "
+        f"phase {tm['phase_sum_ms'] / args.steps:.3f} ms, bc {tm['bc_ms'] / args.steps:.3f} ms")
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_step_jet_d3q27_256x256x1024_fp32.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    out = {
+        "metric": "MLUPS (D3Q27 round jet 256x256x1024, FP32 populations)",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "jet_d3q27_256x256x1024", "velocity_set": "D3Q27", "phase_set": "D3Q7",
+                   "grid": [cfg.nx, cfg.ny, cfg.nz], "nozzle_diameter": cfg.diam, "u_inf": cfg.u_inf,
+                   "reynolds": cfg.reynolds_water, "weber": cfg.weber, "inflow_noise": "uniform +-5% u_z",
+                   "storage": "fp32", "parallelism": "single GPU"},
+        "interior_mlups": round(inner * args.steps / wall / 1e6, 1),
+        "step_roofline_frac": round(mlups * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9), 4),
+        "bytes_per_update": b_alg,
+        "roofline": {"bound": "hbm", "kernel": "k_step", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_launch": kbytes, "avg_launch_ms": round(step_ms, 4)},
+        "kernel_ms": {"phase_sum": round(tm["phase_sum_ms"] / args.steps, 4), "step": round(step_ms, 4),
+                      "jet_bc": round(tm["bc_ms"] / args.steps, 4)},
+        "diagnostics": {"mass": diag["mass"], "phi_mass": diag["phi_mass"], "umax": diag["umax"]},
+    }
+
+    if not args.no_extras:
+        c16 = presets.jet_d3q27_256x256x1024(pf.STORE_FP16)
+        wall16, tm16, _ = run_form(pf, c16, args.steps, args.warmup)
+        mlups16 = cells * args.steps / wall16 / 1e6
+        b16 = presets.bytes_per_update(c16)
+        out["fp16_storage"] = {"value": round(mlups16, 1), "unit": "MLUPS",
+                               "speedup_vs_fp32": round(mlups16 / mlups, 3),
+                               "step_roofline_frac": round(mlups16 * 1e6 * b16 / (HBM_PEAK_GBS * 1e9), 4),
+                               "kernel_ms": round(tm16["step_ms"] / args.steps, 4), "bytes_per_update": b16}
+        log(f"[bench] fp16 storage: {mlups16:.0f} MLUPS ({mlups16 / mlups:.2f}x)")
+        out["laplace"] = [laplace(pf, presets, R) for R in (8, 12, 16)]
+        log(f"[bench] laplace: {[round(r['rel_err'], 4) for r in out['laplace']]}")
+
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: {args.cpu_steps} steps of the oracle")
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,183 @@
+/*
+ * phasefieldlbm.h -- C-ABI of the MI355X phase-field LBM timestep library.
+ *
+ * The reference (brenogemelgo/phaseFieldLBM) has no FFI: its host driver
+ * (src/main.cu:35-215) calls compile-time C++ kernels that take an
+ * `LBMFields` POD of device pointers (src/LBMIncludes.cuh:58-136).  This
+ * header is the boundary that replaces those calls: a host driver in C,
+ * C++ or Python (ctypes) links only against these symbols.  No torch or HIP
+ * types cross it; every pointer argument is caller-owned host memory.
+ *
+ * Every entry point returns PF_OK (0) or a negative pf_status code; the
+ * message of the last failure on a context is available from
+ * pf_last_error().  Contexts are independent; one context is not
+ * thread-safe.
+ */
+#ifndef PHASEFIELDLBM_H
+#define PHASEFIELDLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Compile-time velocity set of the reference (-DVS_D3Q19 | -DVS_D3Q27,
+ * compile.sh:30, src/constants.cuh:39-43).  Each value selects a separate
+ * template instantiation of the kernels. */
+enum pf_velocity_set { PF_D3Q19 = 19, PF_D3Q27 = 27 };
+
+/* Compile-time flow case of the reference (-DJET | -DDROPLET,
+ * src/constants.cuh:45-49, src/flowCase/{Jet,Droplet}.cuh). */
+enum pf_flow_case { PF_JET = 0, PF_DROPLET = 1 };
+
+/* Population storage.  FP32 is the plain path.  FP16 stores the shifted
+ * populations f - w*rho0 and g - w*phi0 in binary16 (round to nearest even)
+ * with FP32 arithmetic, the role of ENABLE_FP16 (src/cuda/precision.cuh:29-42),
+ * which in the reference stores only f in FP16. */
+enum pf_storage { PF_STORE_FP32 = 0, PF_STORE_FP16 = 1 };
+
+/* Inflow perturbation.  GAUSSIAN_XY is the reference's Box-Muller noise with
+ * sigma = 0.08*u_inf on u_x, u_y (src/BoundaryConditions.cuh:60-69).
+ * UNIFORM_Z is u_z = u_inf*(1 + a*(2U-1)) with U drawn from the same integer
+ * hash; it is bit-reproducible between CPU and GPU. */
+enum pf_noise { PF_NOISE_GAUSSIAN_XY = 0, PF_NOISE_UNIFORM_Z = 1, PF_NOISE_NONE = 2 };
+
+/* Curvature closure: finite-difference div(n) (src/phaseField.cuh:142-219)
+ * or the learned MLP at interface cells with the FD value as fallback. */
+enum pf_closure { PF_CLOSURE_FD = 0, PF_CLOSURE_MLP = 1 };
+
+/* Field identifiers; 0..17 follow host::FieldID (src/fileIO/fields.cuh:33-68). */
+enum pf_field {
+    PF_FIELD_RHO = 0, PF_FIELD_UX, PF_FIELD_UY, PF_FIELD_UZ,
+    PF_FIELD_PXX, PF_FIELD_PYY, PF_FIELD_PZZ, PF_FIELD_PXY, PF_FIELD_PXZ, PF_FIELD_PYZ,
+    PF_FIELD_PHI, PF_FIELD_NORMX, PF_FIELD_NORMY, PF_FIELD_NORMZ, PF_FIELD_IND,
+    PF_FIELD_FSX, PF_FIELD_FSY, PF_FIELD_FSZ,
+    PF_NUM_FIELDS
+};
+
+enum pf_status {
+    PF_OK = 0,
+    PF_ERR_CONFIG = -1,    /* invalid configuration or argument */
+    PF_ERR_DEVICE = -2,    /* bad device index / no device */
+    PF_ERR_OOM = -3,       /* device allocation failed */
+    PF_ERR_LAUNCH = -4,    /* kernel launch or runtime error */
+    PF_ERR_IO = -5,        /* file could not be written */
+    PF_ERR_NONFINITE = -6, /* diagnostics found NaN/Inf */
+    PF_ERR_STATE = -7      /* call out of order (e.g. step before init) */
+};
+
+/* pf_step flags */
+#define PF_STEP_SAVE_LAST 1u /* materialise every pf_field on the last step */
+
+/*
+ * Run configuration.  The reference keeps these as constexpr values
+ * (src/constants.cuh:82-141, src/functions/globalFunctions.cuh:260-462);
+ * here they are runtime values so one library serves every grid.
+ * pf_config_default() fills the reference's own JET or DROPLET values.
+ */
+typedef struct pf_config {
+    int32_t velocity_set;  /* pf_velocity_set */
+    int32_t flow_case;     /* pf_flow_case */
+    int32_t storage;       /* pf_storage */
+    int32_t closure;       /* pf_closure */
+    int32_t nx, ny, nz;    /* grid including the 1-cell ghost shell */
+    /* jet (src/constants.cuh:84-109) */
+    int32_t diam;          /* nozzle diameter; radius = diam/2 (integer) */
+    float u_inf;
+    float reynolds_water;
+    float reynolds_oil;
+    float weber;
+    int32_t noise;         /* pf_noise */
+    float noise_amp;       /* UNIFORM_Z relative amplitude a */
+    uint32_t seed;         /* hash base, reference 0x9E3779B9 */
+    /* droplet (src/constants.cuh:113-141, src/initialConditions.cu:58-83) */
+    float radius;
+    float axis_x, axis_y, axis_z; /* coordinate divisors; reference 2,1,1 */
+    float tau;             /* droplet relaxation time */
+    float sigma;           /* droplet surface tension (jet: u^2 D / We) */
+    /* phase field */
+    float interface_width;
+    float tau_g;           /* only sets the sharpening gamma, as in the reference */
+    float phi0;            /* FP16 storage shift of g (g - w*phi0) */
+    /* execution */
+    int32_t device;        /* HIP device index; <0 reads GPU_INDEX like setDeviceFromEnv */
+    int32_t zchunk;        /* z cells marched per workgroup; 0 = automatic */
+    float mlp_lo, mlp_hi;  /* MLP applies where mlp_lo < phi < mlp_hi */
+} pf_config;
+
+typedef struct pf_ctx pf_ctx;
+
+/* Fill `cfg` with the reference defaults of a flow case and velocity set. */
+int pf_config_default(pf_config *cfg, int32_t flow_case, int32_t velocity_set);
+
+/* Allocate every device buffer and the stream.  Replaces setDeviceFromEnv
+ * (src/functions/hostFunctions.cuh:117-150), FieldAllocate
+ * (src/fieldAllocate/FieldAllocate.cuh:47-122) and cudaStreamCreate
+ * (src/main.cu:110-111). */
+int pf_create(const pf_config *cfg, pf_ctx **out);
+void pf_destroy(pf_ctx *ctx);
+
+/* Initial conditions: setInitialDensity, setJet|setDroplet, setDistros
+ * (src/main.cu:114-116, src/initialConditions.cu:33-138). */
+int pf_init(pf_ctx *ctx);
+
+/* Replace field `id` (PF_FIELD_RHO..UZ or PHI) with host data of n =
+ * nx*ny*nz floats; follow with pf_init_distros.  Used to start from an
+ * arbitrary state (host-computed ICs). */
+int pf_set_field(pf_ctx *ctx, int32_t id, const float *host, size_t n);
+/* setDistros only (src/initialConditions.cu:101-138): populations at
+ * equilibrium with the current rho, u, phi fields. */
+int pf_init_distros(pf_ctx *ctx);
+
+/* Advance nsteps timesteps; step s uses time index t0+s for the inflow
+ * noise.  One step = graph (computePhase, computeNormals, computeForces,
+ * computeMoments, streamCollide) + flow-case boundary conditions, as in
+ * src/main.cu:163-172 and src/cuda/CUDAGraph.cuh:44-50.  Asynchronous. */
+int pf_step(pf_ctx *ctx, uint64_t t0, uint32_t nsteps, uint32_t flags);
+int pf_sync(pf_ctx *ctx);
+
+/* Copy field `id` (n = nx*ny*nz floats) to host memory.  Fields other than
+ * PHI hold the values of the last step run with PF_STEP_SAVE_LAST (or the
+ * initial conditions), as the reference's arrays do after that step
+ * (PostProcess::bin, src/postProcess/PostProcess.cuh:43-97). */
+int pf_get_field(pf_ctx *ctx, int32_t id, float *host, size_t n);
+
+/* Copy the post-streaming populations f (which=0, Q*N) or g (which=1, 7*N)
+ * in the reference's layout f[q*N + idx], unshifted for g and shifted by -w
+ * for f (the reference's stored value), as FP32. */
+int pf_get_populations(pf_ctx *ctx, int32_t which, float *host, size_t n);
+
+/* FP64 reductions over interior cells of the last saved fields:
+ * out[0]=sum rho, out[1..3]=sum rho*u, out[4]=sum phi, out[5]=max|u|.
+ * Returns PF_ERR_NONFINITE if any value is not finite. */
+int pf_diagnostics(pf_ctx *ctx, double out[6]);
+
+/* FP64 sum of phi over interior cells, from the current populations (no
+ * save needed): the phase mass. */
+int pf_phase_mass(pf_ctx *ctx, double *mass);
+
+/* Learned-closure weights, fp32 host arrays in row-major [out][in]:
+ * W1[64*27], b1[64], W2[64*64], b2[64], W3[64], b3[1].  Stored on the
+ * device as BF16 weights and FP32 biases. */
+int pf_set_closure(pf_ctx *ctx, const float *W1, const float *b1, const float *W2,
+                   const float *b2, const float *W3, const float *b3);
+
+/* Time `nsteps` steps (flags as pf_step) with HIP events on the context's
+ * stream.  out_ms[0] = total, out_ms[1] = phase-sum kernel total,
+ * out_ms[2] = fused collide-stream kernel total, out_ms[3] = boundary total.
+ * Per-kernel times are measured in a second, separately-evented pass. */
+int pf_time_steps(pf_ctx *ctx, uint64_t t0, uint32_t nsteps, float out_ms[4]);
+
+/* Bytes of device memory held by the context. */
+size_t pf_device_bytes(const pf_ctx *ctx);
+
+const char *pf_last_error(const pf_ctx *ctx);
+const char *pf_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHASEFIELDLBM_H */

@@ -1,0 +1,206 @@
+"""MI355X-native phase-field lattice Boltzmann timestep (phaseFieldLBM drop-in).
+
+The host side mirrors the reference's driver interface (src/main.cu:35-215):
+a ``Simulation`` owns the fields of one run, ``init()`` applies the initial
+conditions, ``step()`` advances the timestep (phase field, moments,
+collide-stream, flow-case boundary conditions) and ``field()`` reads the
+macroscopic arrays named as in host::FieldID (src/fileIO/fields.cuh:33-68).
+All work runs in the HIP library ``libphasefieldlbm_amd.so`` through its
+C-ABI; nothing here computes physics.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, fields as dc_fields
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import (CLOSURE_FD, CLOSURE_MLP, D3Q19, D3Q27, DROPLET, FIELDS, JET, NOISE_GAUSSIAN_XY,
+                   NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, STORE_FP32)
+
+__all__ = ["Config", "Simulation", "PfError", "presets", "FIELDS", "D3Q19", "D3Q27", "JET", "DROPLET",
+           "STORE_FP32", "STORE_FP16", "NOISE_GAUSSIAN_XY", "NOISE_UNIFORM_Z", "NOISE_NONE",
+           "CLOSURE_FD", "CLOSURE_MLP", "library_path"]
+
+
+def library_path() -> str:
+    return _abi.LIB_PATH
+
+
+class PfError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"phasefieldlbm status {status} ({_abi.STATUS.get(status, '?')}): {msg}")
+        self.status = status
+
+
+@dataclass
+class Config:
+    """Run configuration; the fields of ``pf_config`` (include/phasefieldlbm.h)."""
+    velocity_set: int = D3Q19
+    flow_case: int = JET
+    storage: int = STORE_FP32
+    closure: int = CLOSURE_FD
+    nx: int = 64
+    ny: int = 64
+    nz: int = 128
+    diam: int = 10
+    u_inf: float = 0.05
+    reynolds_water: float = 5000.0
+    reynolds_oil: float = 5000.0
+    weber: float = 500.0
+    noise: int = NOISE_GAUSSIAN_XY
+    noise_amp: float = 0.05
+    seed: int = 0x9E3779B9
+    radius: float = 35.0
+    axis_x: float = 2.0
+    axis_y: float = 1.0
+    axis_z: float = 1.0
+    tau: float = 0.55
+    sigma: float = 0.03
+    interface_width: float = 4.0
+    tau_g: float = 1.0
+    phi0: float = 0.0
+    device: int = -1
+    zchunk: int = 0
+    mlp_lo: float = 0.05
+    mlp_hi: float = 0.95
+
+    @classmethod
+    def reference(cls, flow_case: int, velocity_set: int) -> "Config":
+        """The reference's compile-time defaults (src/constants.cuh:82-141)."""
+        lib = _abi.load()
+        c = _abi.PfConfig()
+        rc = lib.pf_config_default(ctypes.byref(c), flow_case, velocity_set)
+        if rc:
+            raise PfError(rc, "pf_config_default")
+        return cls(**{f.name: getattr(c, f.name) for f in dc_fields(cls)})
+
+    def to_c(self) -> _abi.PfConfig:
+        c = _abi.PfConfig()
+        for f in dc_fields(self):
+            setattr(c, f.name, getattr(self, f.name))
+        return c
+
+    @property
+    def shape(self):
+        """numpy shape of a field: (nz, ny, nx), x fastest."""
+        return (self.nz, self.ny, self.nx)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def q(self) -> int:
+        return self.velocity_set
+
+
+class Simulation:
+    """One simulation on one GPU (the reference's LBMFields + stream)."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self._lib = _abi.load()
+        self._c = cfg.to_c()
+        h = ctypes.c_void_p()
+        rc = self._lib.pf_create(ctypes.byref(self._c), ctypes.byref(h))
+        if rc:
+            raise PfError(rc, "pf_create")
+        self._h = h
+        self.t = 0
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str):
+        if rc:
+            msg = self._lib.pf_last_error(self._h)
+            raise PfError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    # -- reference driver surface -------------------------------------------
+    def init(self):
+        """setInitialDensity + setJet|setDroplet + setDistros."""
+        self._check(self._lib.pf_init(self._h), "pf_init")
+        self.t = 0
+        return self
+
+    def set_field(self, name: str, values: np.ndarray):
+        a = np.ascontiguousarray(values, dtype=np.float32).reshape(-1)
+        self._check(self._lib.pf_set_field(self._h, _abi.FIELD_ID[name], a.ctypes.data, a.size),
+                    "pf_set_field")
+
+    def init_distros(self):
+        self._check(self._lib.pf_init_distros(self._h), "pf_init_distros")
+        self.t = 0
+
+    def step(self, n: int = 1, save: bool = False, t0: Optional[int] = None):
+        """Advance n timesteps; the inflow noise uses time index t0+s."""
+        t0 = self.t if t0 is None else t0
+        self._check(self._lib.pf_step(self._h, t0, n, _abi.STEP_SAVE_LAST if save else 0), "pf_step")
+        self.t = t0 + n
+        return self
+
+    def sync(self):
+        self._check(self._lib.pf_sync(self._h), "pf_sync")
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.empty(self.cfg.cells, dtype=np.float32)
+        self._check(self._lib.pf_get_field(self._h, _abi.FIELD_ID[name], out.ctypes.data, out.size),
+                    "pf_get_field")
+        return out.reshape(self.cfg.shape)
+
+    def populations(self, which: str = "f") -> np.ndarray:
+        nq = self.cfg.q if which == "f" else 7
+        out = np.empty(nq * self.cfg.cells, dtype=np.float32)
+        self._check(self._lib.pf_get_populations(self._h, 0 if which == "f" else 1, out.ctypes.data,
+                                                 out.size), "pf_get_populations")
+        return out.reshape((nq,) + self.cfg.shape)
+
+    def diagnostics(self) -> dict:
+        out = np.zeros(6, dtype=np.float64)
+        self._check(self._lib.pf_diagnostics(self._h, out.ctypes.data), "pf_diagnostics")
+        return {"mass": out[0], "momentum": out[1:4].copy(), "phi_mass": out[4], "umax": out[5]}
+
+    def phase_mass(self) -> float:
+        m = ctypes.c_double()
+        self._check(self._lib.pf_phase_mass(self._h, ctypes.byref(m)), "pf_phase_mass")
+        return m.value
+
+    def set_closure(self, W1, b1, W2, b2, W3, b3):
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (W1, b1, W2, b2, W3, b3)]
+        sizes = [64 * 27, 64, 64 * 64, 64, 64, 1]
+        for a, n in zip(arrs, sizes):
+            if a.size != n:
+                raise ValueError(f"closure array of size {a.size}, expected {n}")
+        self._check(self._lib.pf_set_closure(self._h, *[a.ctypes.data for a in arrs]), "pf_set_closure")
+
+    def time_steps(self, n: int) -> dict:
+        """HIP-event timing of n steps on the simulation's stream (ms)."""
+        out = np.zeros(4, dtype=np.float32)
+        self._check(self._lib.pf_time_steps(self._h, self.t, n, out.ctypes.data), "pf_time_steps")
+        self.t += 2 * n
+        return {"total_ms": float(out[0]), "phase_sum_ms": float(out[1]),
+                "step_ms": float(out[2]), "bc_ms": float(out[3])}
+
+    def device_bytes(self) -> int:
+        return int(self._lib.pf_device_bytes(self._h))
+
+
+from . import presets  # noqa: E402

@@ -1,0 +1,275 @@
+// Initial conditions, population gather and FP64 reductions.
+//
+//   k_init_fields    setInitialDensity + setJet (src/initialConditions.cu:33-99);
+//                    the droplet profile is computed on the host
+//   k_init_distros   setDistros (src/initialConditions.cu:101-138), written in
+//                    the layout the first (even) A-A step reads
+//   k_gather         post-streaming populations in the reference layout
+//   k_diagnostics    FP64 sums of rho, rho*u, phi and max|u| (no reference
+//                    counterpart; the reference prints no diagnostics)
+//   k_phase_mass     FP64 sum of phi = sum_q g_q straight from the populations
+#include <hip/hip_runtime.h>
+
+#include "kparams.hpp"
+#include "lattice.hpp"
+
+namespace pf {
+
+template <bool HALF>
+__device__ __forceinline__ void st_pop(void* base, uint32_t N, int slot, uint32_t i, float v) {
+    if constexpr (HALF)
+        reinterpret_cast<_Float16*>(base)[(size_t)slot * N + i] = (_Float16)v;
+    else
+        reinterpret_cast<float*>(base)[(size_t)slot * N + i] = v;
+}
+template <bool HALF>
+__device__ __forceinline__ float ld_pop(const void* base, uint32_t N, int slot, uint32_t i) {
+    if constexpr (HALF)
+        return (float)reinterpret_cast<const _Float16*>(base)[(size_t)slot * N + i];
+    else
+        return reinterpret_cast<const float*>(base)[(size_t)slot * N + i];
+}
+
+__global__ __launch_bounds__(256) void k_init_fields(KParams p, int jet) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.N) return;
+    p.fld[0][i] = 1.0f;
+    if (jet) {
+        const uint32_t plane = (uint32_t)p.nx * (uint32_t)p.ny;
+        if (i < plane) {
+#pragma clang fp contract(off)
+            const int x = (int)(i % (uint32_t)p.nx), y = (int)(i / (uint32_t)p.nx);
+            const float dx = (float)x - p.cxg, dy = (float)y - p.cyg;
+            if (dx * dx + dy * dy <= p.R2) {
+                p.fld[10][i] = 1.0f;
+                p.fld[3][i] = p.u_inf;
+            }
+        }
+    }
+}
+
+template <int Q, bool HALF>
+__global__ __launch_bounds__(256) void k_init_distros(KParams p) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.N) return;
+    const float rho = p.fld[0][i], ux = p.fld[1][i], uy = p.fld[2][i], uz = p.fld[3][i];
+    const float phi = p.fld[10][i];
+    const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+    static_for<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        st_pop<HALF>(p.f, p.N, q, i, f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
+    });
+    static_for<7>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        float v = g_post<q>(phi, ux, uy, uz, 0.0f, 0.0f, 0.0f, 0.0f);
+        if constexpr (HALF) v -= p.wphi0[q];
+        st_pop<HALF>(p.g, p.N, q, i, v);
+    });
+}
+
+// which = 0: f (Q slots, stored shift -w kept); which = 1: g (7 slots, unshifted)
+template <int Q, bool JET, bool HALF, int PAR>
+__global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.N) return;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int x = (int)(i % (uint32_t)nx);
+    const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
+    const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+    const bool interior = x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && z >= 1 && z <= nz - 2;
+    const int xm = (x == 1) ? nx - 2 : x - 1, xp = (x == nx - 2) ? 1 : x + 1;
+    const int ym = (y == 1) ? ny - 2 : y - 1, yp = (y == ny - 2) ? 1 : y + 1;
+    int zm = z - 1, zp = z + 1;
+    if constexpr (!JET) {
+        zm = (z == 1) ? nz - 2 : z - 1;
+        zp = (z == nz - 2) ? 1 : z + 1;
+    }
+    const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp}, zs[3] = {zm, z, zp};
+    const int nq = which == 0 ? Q : 7;
+    for (int q = 0; q < nq; ++q) {
+        float v = 0.0f;
+        if (interior) {
+            uint32_t src = i;
+            int slot = q;
+            if constexpr (PAR == 1) {
+                slot = opp(q);
+                src = (uint32_t)xs[1 - kCX[q]] +
+                      (uint32_t)nx * ((uint32_t)ys[1 - kCY[q]] + (uint32_t)ny * (uint32_t)zs[1 - kCZ[q]]);
+            }
+            if (which == 0) {
+                v = ld_pop<HALF>(p.f, p.N, slot, src);
+            } else {
+                v = ld_pop<HALF>(p.g, p.N, slot, src);
+                if constexpr (HALF) v += p.wphi0[q];
+            }
+        }
+        out[(size_t)q * p.N + i] = v;
+    }
+}
+
+// Block reduction of NV doubles (sum, except the last, which is max when
+// LASTMAX).  256 threads = 4 wavefronts of 64.
+template <int NV, bool LASTMAX>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double* out) {
+    __shared__ double s[4][NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double a = v[k];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double b = __shfl_xor(a, off, 64);
+            a = (LASTMAX && k == NV - 1) ? fmax(a, b) : a + b;
+        }
+        v[k] = a;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s[wid][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double a = s[0][k];
+            for (int w = 1; w < 4; ++w) a = (LASTMAX && k == NV - 1) ? fmax(a, s[w][k]) : a + s[w][k];
+            out[k] = a;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diagnostics(KParams p, double* partials) {
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < p.N; i += gridDim.x * 256u) {
+        const int x = (int)(i % (uint32_t)nx);
+        const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
+        const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+        if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
+        const double rho = p.fld[0][i], ux = p.fld[1][i], uy = p.fld[2][i], uz = p.fld[3][i];
+        v[0] += rho;
+        v[1] += rho * ux;
+        v[2] += rho * uy;
+        v[3] += rho * uz;
+        v[4] += (double)p.fld[10][i];
+        const double um = sqrt(ux * ux + uy * uy + uz * uz);
+        if (v[5] == v[5]) v[5] = (um == um) ? fmax(v[5], um) : um;  // a NaN, once seen, stays
+    }
+    block_reduce<6, true>(v, partials + (size_t)blockIdx.x * 6);
+}
+
+template <bool JET, bool HALF, int PAR>
+__global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials) {
+    double v[1] = {0.0};
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < p.N; i += gridDim.x * 256u) {
+        const int x = (int)(i % (uint32_t)nx);
+        const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
+        const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+        if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
+        const int xm = (x == 1) ? nx - 2 : x - 1, xp = (x == nx - 2) ? 1 : x + 1;
+        const int ym = (y == 1) ? ny - 2 : y - 1, yp = (y == ny - 2) ? 1 : y + 1;
+        int zm = z - 1, zp = z + 1;
+        if constexpr (!JET) {
+            zm = (z == 1) ? nz - 2 : z - 1;
+            zp = (z == nz - 2) ? 1 : z + 1;
+        }
+        const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp}, zs[3] = {zm, z, zp};
+        double phi = 0.0;
+        for (int q = 0; q < 7; ++q) {
+            uint32_t src = i;
+            int slot = q;
+            if constexpr (PAR == 1) {
+                slot = opp(q);
+                src = (uint32_t)xs[1 - kCX[q]] +
+                      (uint32_t)nx * ((uint32_t)ys[1 - kCY[q]] + (uint32_t)ny * (uint32_t)zs[1 - kCZ[q]]);
+            }
+            float gv = ld_pop<HALF>(p.g, p.N, slot, src);
+            if constexpr (HALF) gv += p.wphi0[q];
+            phi += (double)gv;
+        }
+        v[0] += phi;
+    }
+    block_reduce<1, false>(v, partials + blockIdx.x);
+}
+
+// Final pass over per-block partials: sums, except value 5 of a 6-wide
+// diagnostics record, which is a max.
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* partials, int nblocks, int nvals,
+                                                          double* out) {
+    for (int k = 0; k < nvals; ++k) {
+        const bool is_max = (nvals == 6 && k == 5);
+        double a = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) {
+            const double x = partials[(size_t)b * nvals + k];
+            a = is_max ? ((x == x) ? fmax(a, x) : x) : a + x;
+        }
+        double v[1] = {a};
+        if (is_max)
+            block_reduce<1, true>(v, out + k);
+        else
+            block_reduce<1, false>(v, out + k);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_init_fields(const KParams& p, bool jet, float, float, float, float, float, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_fields, dim3((p.N + 255) / 256), dim3(256), 0, s, p, jet ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_distros(const KParams& p, int Q, bool half, hipStream_t s) {
+    const dim3 grid((p.N + 255) / 256);
+    if (Q == 27) {
+        if (half) hipLaunchKernelGGL((k_init_distros<27, true>), grid, dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((k_init_distros<27, false>), grid, dim3(256), 0, s, p);
+    } else {
+        if (half) hipLaunchKernelGGL((k_init_distros<19, true>), grid, dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((k_init_distros<19, false>), grid, dim3(256), 0, s, p);
+    }
+    return hipGetLastError();
+}
+
+template <int Q, bool JET, bool HALF>
+static hipError_t gather_t(const KParams& p, int parity, int which, float* out, hipStream_t s) {
+    const dim3 grid((p.N + 255) / 256);
+    if (parity) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 1>), grid, dim3(256), 0, s, p, which, out);
+    else hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0>), grid, dim3(256), 0, s, p, which, out);
+    return hipGetLastError();
+}
+
+template <int Q>
+static hipError_t gather_q(const KParams& p, bool jet, bool half, int parity, int which, float* out, hipStream_t s) {
+    if (jet) return half ? gather_t<Q, true, true>(p, parity, which, out, s) : gather_t<Q, true, false>(p, parity, which, out, s);
+    return half ? gather_t<Q, false, true>(p, parity, which, out, s) : gather_t<Q, false, false>(p, parity, which, out, s);
+}
+
+hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, int which, float* out,
+                         hipStream_t s) {
+    return Q == 27 ? gather_q<27>(p, jet, half, parity, which, out, s) : gather_q<19>(p, jet, half, parity, which, out, s);
+}
+
+hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_diagnostics, dim3(nblocks), dim3(256), 0, s, p, partials);
+    return hipGetLastError();
+}
+
+template <bool JET, bool HALF>
+static hipError_t mass_t(const KParams& p, int parity, double* partials, int nblocks, hipStream_t s) {
+    if (parity) hipLaunchKernelGGL((k_phase_mass<JET, HALF, 1>), dim3(nblocks), dim3(256), 0, s, p, partials);
+    else hipLaunchKernelGGL((k_phase_mass<JET, HALF, 0>), dim3(nblocks), dim3(256), 0, s, p, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_mass(const KParams& p, bool jet, bool half, int parity, double* partials, int nblocks,
+                             hipStream_t s) {
+    if (jet) return half ? mass_t<true, true>(p, parity, partials, nblocks, s) : mass_t<true, false>(p, parity, partials, nblocks, s);
+    return half ? mass_t<false, true>(p, parity, partials, nblocks, s) : mass_t<false, false>(p, parity, partials, nblocks, s);
+}
+
+hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, partials, nblocks, nvals, out);
+    return hipGetLastError();
+}
+
+}  // namespace pf

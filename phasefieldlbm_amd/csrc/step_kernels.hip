@@ -1,0 +1,604 @@
+// One timestep of the phase-field LBM on gfx950: three kernels.
+//
+//   k_phase_sum  phi = sum_q g_q                         (src/phaseField.cuh:31-52)
+//   k_step       fused normals + curvature + CSF force + moments +
+//                regularized collision + f and g streaming
+//                (src/phaseField.cuh:54-219, src/lbm.cu:33-239)
+//   k_jet_bc     inflow and outflow planes of the jet
+//                (src/BoundaryConditions.cuh:36-169)
+//
+// Streaming is in place with the A-A pattern: one population buffer, two
+// alternating step kinds.  With S[slot][cell]:
+//   even step  in_q  = S[q][x]              out_q -> S[opp q][x]
+//   odd  step  in_q  = S[opp q][x - c_q]    out_q -> S[q][x + c_q]
+// Each cell reads and writes the same set of locations, so no other cell
+// touches them within a step.  Neighbour coordinates wrap with the
+// reference's ghost-shell periodicity (0 -> n-2, n-1 -> 1,
+// src/functions/deviceFunctions.cuh:63-76): x and y for the jet, x, y and z
+// for the droplet.  The jet's z = 0 and z = nz-1 ghost planes carry the
+// populations the boundary kernel supplies for the next step.
+//
+// k_step marches a 64 x 4 column of cells through z.  A 5-plane ring of the
+// phi tile with a 2-cell halo and a 3-plane ring of normals with a 1-cell
+// halo live in LDS, so phi is read from HBM about once per cell and the
+// normals never leave the chip (the reference stores normx/y/z, ind and the
+// force in HBM between three kernels).
+#include <hip/hip_runtime.h>
+
+#include "kparams.hpp"
+#include "lattice.hpp"
+#include "mlp_device.hpp"
+
+namespace pf {
+
+constexpr int kHX = kTX + 4;  // phi tile with 2-cell halo
+constexpr int kHY = kTY + 4;
+constexpr int kNX1 = kTX + 2;  // normal tile with 1-cell halo
+constexpr int kNY1 = kTY + 2;
+
+// ---------------------------------------------------------------------------
+// population storage
+// ---------------------------------------------------------------------------
+template <bool HALF>
+struct Pop;
+template <>
+struct Pop<false> {
+    using T = float;
+    static __device__ __forceinline__ float ld(const void* base, uint32_t N, int slot, uint32_t i) {
+        const char* b = reinterpret_cast<const char*>(base) + (size_t)slot * N * 4u;
+        return *reinterpret_cast<const float*>(b + (i << 2));
+    }
+    static __device__ __forceinline__ void st(void* base, uint32_t N, int slot, uint32_t i, float v) {
+        char* b = reinterpret_cast<char*>(base) + (size_t)slot * N * 4u;
+        *reinterpret_cast<float*>(b + (i << 2)) = v;
+    }
+};
+template <>
+struct Pop<true> {
+    using T = _Float16;
+    static __device__ __forceinline__ float ld(const void* base, uint32_t N, int slot, uint32_t i) {
+        const char* b = reinterpret_cast<const char*>(base) + (size_t)slot * N * 2u;
+        return (float)*reinterpret_cast<const _Float16*>(b + (i << 1));
+    }
+    static __device__ __forceinline__ void st(void* base, uint32_t N, int slot, uint32_t i, float v) {
+        char* b = reinterpret_cast<char*>(base) + (size_t)slot * N * 2u;
+        *reinterpret_cast<_Float16*>(b + (i << 1)) = (_Float16)v;
+    }
+};
+
+// Neighbour offsets of one cell: xw[dx+1] + row[dy+1][dz+1].
+struct Nbr {
+    uint32_t xw[3];
+    uint32_t row[3][3];
+};
+
+template <bool WRAPZ>
+__device__ __forceinline__ Nbr make_nbr(int x, int y, int z, int nx, int ny, int nz) {
+    Nbr n;
+    const int xm = (x == 1) ? nx - 2 : x - 1, xp = (x == nx - 2) ? 1 : x + 1;
+    const int ym = (y == 1) ? ny - 2 : y - 1, yp = (y == ny - 2) ? 1 : y + 1;
+    int zm = z - 1, zp = z + 1;
+    if constexpr (WRAPZ) {
+        zm = (z == 1) ? nz - 2 : z - 1;
+        zp = (z == nz - 2) ? 1 : z + 1;
+    }
+    n.xw[0] = (uint32_t)xm;
+    n.xw[1] = (uint32_t)x;
+    n.xw[2] = (uint32_t)xp;
+    const int yy[3] = {ym, y, yp}, zz[3] = {zm, z, zp};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) n.row[a][b] = (uint32_t)nx * ((uint32_t)yy[a] + (uint32_t)ny * (uint32_t)zz[b]);
+    return n;
+}
+
+template <int q, int SIGN>
+__device__ __forceinline__ uint32_t nb_idx(const Nbr& n) {
+    return n.xw[SIGN * kCX[q] + 1] + n.row[SIGN * kCY[q] + 1][SIGN * kCZ[q] + 1];
+}
+
+// Where the next step of parity PAR reads in_q of the cell with neighbour
+// table n (cell index c).
+template <int q, int PAR>
+__device__ __forceinline__ void in_slot(const Nbr& n, uint32_t c, int& slot, uint32_t& idx) {
+    if constexpr (PAR == 0) {
+        slot = q;
+        idx = c;
+    } else {
+        slot = opp(q);
+        idx = nb_idx<q, -1>(n);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_phase_sum: phi = sum_q g_q over interior cells
+// ---------------------------------------------------------------------------
+template <bool JET, bool HALF, int PAR>
+__global__ __launch_bounds__(kThreads) void k_phase_sum(KParams p) {
+    const int x = blockIdx.x * kTX + (threadIdx.x & (kTX - 1));
+    const int y = blockIdx.y * kTY + (threadIdx.x >> 6);
+    const int z = blockIdx.z + 1;
+    if (x < 1 || x > p.nx - 2 || y < 1 || y > p.ny - 2) return;
+    const uint32_t c = (uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z);
+    const Nbr n = make_nbr<!JET>(x, y, z, p.nx, p.ny, p.nz);
+    float g[7];
+    static_for<7>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        int slot;
+        uint32_t i;
+        in_slot<q, PAR>(n, c, slot, i);
+        g[q] = Pop<HALF>::ld(p.g, p.N, slot, i);
+        if constexpr (HALF) g[q] += p.wphi0[q];
+    });
+    float phi = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 7; ++q) phi += g[q];
+    p.fld[10][c] = phi;
+}
+
+// ---------------------------------------------------------------------------
+// k_step: fused phase-field forcing + regularized collide-stream
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblk) {
+    // Workgroups are dealt round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of tiles so x-adjacent tiles share its L2.
+    const uint32_t q = nblk >> 3, r = nblk & 7u;
+    const uint32_t k = b & 7u;
+    return k * q + min(k, r) + (b >> 3);
+}
+
+template <int Q, bool JET, bool HALF, int PAR, bool MLP>
+__global__ __launch_bounds__(kThreads) void k_step(KParams p) {
+    __shared__ float s_phi[5][kHY * kHX];
+    __shared__ float s_n[3][4][kNY1 * kNX1];
+    __shared__ MlpShared<MLP> s_mlp;
+
+    const int tid = threadIdx.x;
+    const int tx = tid & (kTX - 1), ty = tid >> 6;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int bx = (int)(lb % (uint32_t)p.ntx);
+    const uint32_t rem = lb / (uint32_t)p.ntx;
+    const int by = (int)(rem % (uint32_t)p.nty);
+    const int bz = (int)(rem / (uint32_t)p.nty);
+    const int x0 = bx * kTX, y0 = by * kTY;
+    const int zs = 1 + bz * p.zchunk;
+    const int ze = min(zs + p.zchunk, nz - 1);
+    const float* __restrict__ phi_g = p.fld[10];
+
+    if constexpr (MLP) mlp_load_weights(s_mlp, p, tid);
+
+    auto load_phi = [&](int pz) {
+        float* dst = s_phi[(pz + 5) % 5];
+        for (int i = tid; i < kHX * kHY; i += kThreads) {
+            const int lx = i % kHX, ly = i / kHX;
+            const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+            float v = 0.0f;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && pz >= 0 && pz < nz)
+                v = phi_g[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)pz)];
+            dst[i] = v;
+        }
+    };
+
+    // normals and |grad phi| on the 1-halo tile of plane pz
+    // (src/phaseField.cuh:54-140); zero at ghost cells, which the reference
+    // never writes.
+    auto comp_normals = [&](int pz) {
+        const int r = pz % 3;
+        const float* pm = s_phi[(pz + 4) % 5];
+        const float* p0 = s_phi[pz % 5];
+        const float* pp = s_phi[(pz + 1) % 5];
+        for (int i = tid; i < kNX1 * kNY1; i += kThreads) {
+            const int lx = i % kNX1, ly = i / kNX1;
+            const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+            float nrx = 0.0f, nry = 0.0f, nrz = 0.0f, ind = 0.0f;
+            if (gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2 && pz >= 1 && pz <= nz - 2) {
+                const int o = (ly + 1) * kHX + (lx + 1);
+                auto val = [&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    constexpr int off = kCY[q] * kHX + kCX[q];
+                    const float* pl = kCZ[q] < 0 ? pm : (kCZ[q] > 0 ? pp : p0);
+                    return pl[o + off];
+                };
+                const float gx_ = kAs2 * iso_sum<Q, 0>(val);
+                const float gy_ = kAs2 * iso_sum<Q, 1>(val);
+                const float gz_ = kAs2 * iso_sum<Q, 2>(val);
+                ind = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_);
+                const float inv = 1.0f / (ind + 1e-9f);
+                nrx = gx_ * inv;
+                nry = gy_ * inv;
+                nrz = gz_ * inv;
+            }
+            s_n[r][0][i] = nrx;
+            s_n[r][1][i] = nry;
+            s_n[r][2][i] = nrz;
+            s_n[r][3][i] = ind;
+        }
+    };
+
+    // prologue
+    for (int pz = zs - 2; pz <= zs + 1; ++pz) load_phi(pz);
+    __syncthreads();
+    comp_normals(zs - 1);
+    comp_normals(zs);
+
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2);
+
+    for (int z = zs; z < ze; ++z) {
+        load_phi(z + 2);
+        __syncthreads();
+        comp_normals(z + 1);
+        __syncthreads();
+
+        // ---- curvature and continuum surface force (src/phaseField.cuh:142-219)
+        const int o = (ty + 1) * kNX1 + (tx + 1);
+        const float* nm[3] = {s_n[(z + 2) % 3][0], s_n[z % 3][0], s_n[(z + 1) % 3][0]};
+        auto nval = [&](int comp_, auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int off = kCY[q] * kNX1 + kCX[q];
+            return nm[kCZ[q] + 1][comp_ * (kNY1 * kNX1) + o + off];
+        };
+        const float phi = s_phi[z % 5][(ty + 2) * kHX + (tx + 2)];
+        float curvature = 0.0f;
+        if (active) {
+            const float scx = iso_sum<Q, 0>([&](auto qc) { return nval(0, qc); });
+            const float scy = iso_sum<Q, 1>([&](auto qc) { return nval(1, qc); });
+            const float scz = iso_sum<Q, 2>([&](auto qc) { return nval(2, qc); });
+            curvature = kAs2 * (scx + scy + scz);
+        }
+        if constexpr (MLP) {
+            curvature = mlp_curvature(s_mlp, p, s_phi, z, tx, ty, tid, active, phi, curvature);
+        }
+        if (!active) continue;
+
+        const float ind = s_n[z % 3][3][o];
+        const float nrx = s_n[z % 3][0][o], nry = s_n[z % 3][1][o], nrz = s_n[z % 3][2][o];
+        const float stCurv = -p.sigma * curvature * ind;
+        const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
+
+        // ---- moments (src/lbm.cu:33-128)
+        const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+        const Nbr nb = make_nbr<!JET>(x, y, z, nx, ny, nz);
+        float pop[Q];
+        static_for<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (PAR == 0) {
+                pop[q] = Pop<HALF>::ld(p.f, p.N, q, c);
+            } else {
+                pop[q] = Pop<HALF>::ld(p.f, p.N, opp(q), nb_idx<q, -1>(nb));
+            }
+        });
+        float rho = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) rho += pop[q];
+        rho += 1.0f;
+        const float invRho = 1.0f / rho;
+        auto pv = [&](auto qc) { return pop[decltype(qc)::value]; };
+        float ux = invRho * pair_chain<Q, 0, -1>(pv);
+        float uy = invRho * pair_chain<Q, 1, -1>(pv);
+        float uz = invRho * pair_chain<Q, 2, -1>(pv);
+        ux += fsx * 0.5f * invRho;
+        uy += fsy * 0.5f * invRho;
+        uz += fsz * 0.5f * invRho;
+        const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+        float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
+        static_for<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const float cu = cu_of<Q, q>(ux, uy, uz);
+            const float feq = f_eq<Q, q>(rho, uu, cu);
+            const float frc = force_of<Q, q>(cu, ux, uy, uz, fsx, fsy, fsz);
+            const float fneq = pop[q] - feq + 0.5f * frc;
+            if constexpr (kCX[q] != 0) pxx += fneq;
+            if constexpr (kCY[q] != 0) pyy += fneq;
+            if constexpr (kCZ[q] != 0) pzz += fneq;
+            if constexpr (kCX[q] * kCY[q] > 0) pxy += fneq;
+            if constexpr (kCX[q] * kCY[q] < 0) pxy -= fneq;
+            if constexpr (kCX[q] * kCZ[q] > 0) pxz += fneq;
+            if constexpr (kCX[q] * kCZ[q] < 0) pxz -= fneq;
+            if constexpr (kCY[q] * kCZ[q] > 0) pyz += fneq;
+            if constexpr (kCY[q] * kCZ[q] < 0) pyz -= fneq;
+        });
+        const float trace = pxx + pyy + pzz;
+        pxx -= kCs2 * trace;
+        pyy -= kCs2 * trace;
+        pzz -= kCs2 * trace;
+
+        // ---- relaxation (src/lbm.cu:158-172)
+        float omega;
+        if constexpr (JET) {
+            const float tau_phi = (1.0f - phi) * p.tau_water + phi * p.tau_oil;
+            const float zn = (float)z * p.sp_inv_nz_m1;
+            float s = (zn - p.sp_z_start) * p.sp_inv_sponge;
+            s = fminf(fmaxf(s, 0.0f), 1.0f);
+            const float r = s * s * (3.0f - 2.0f * s);
+            const float visc_local = (1.0f - phi) * p.visc_water + phi * p.visc_oil;
+            const float tau_zmax = 0.5f + kAs2 * (visc_local * (1.0f + 100.0f));
+            const float tau_eff = tau_phi + r * (tau_zmax - tau_phi);
+            omega = 1.0f / tau_eff;
+        } else {
+            omega = p.omega_ref;
+        }
+        const float omco = 1.0f - omega;
+
+        // ---- collide and stream f (src/lbm.cu:176-207)
+        static_for<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const float cu = cu_of<Q, q>(ux, uy, uz);
+            const float feq = f_eq<Q, q>(rho, uu, cu);
+            const float frc = force_of<Q, q>(cu, ux, uy, uz, fsx, fsy, fsz);
+            const float fneq = f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, ux, uy, uz);
+            const float fo = feq + omco * fneq + 0.5f * frc;
+            if constexpr (PAR == 0) {
+                Pop<HALF>::st(p.f, p.N, opp(q), c, fo);
+            } else {
+                Pop<HALF>::st(p.f, p.N, q, nb_idx<q, 1>(nb), fo);
+            }
+        });
+
+        // ---- phase populations (src/lbm.cu:209-238)
+        const float sharp = p.gamma * phi * (1.0f - phi);
+        static_for<7>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            float go = g_post<q>(phi, ux, uy, uz, sharp, nrx, nry, nrz);
+            if constexpr (HALF) go -= p.wphi0[q];
+            if constexpr (PAR == 0) {
+                Pop<HALF>::st(p.g, p.N, opp(q), c, go);
+            } else {
+                Pop<HALF>::st(p.g, p.N, q, nb_idx<q, 1>(nb), go);
+            }
+        });
+
+        // ---- macroscopic fields for output and for the jet boundary planes
+        if (p.save) {
+            float* const* F = p.fld;
+            F[0][c] = rho; F[1][c] = ux; F[2][c] = uy; F[3][c] = uz;
+            F[4][c] = pxx; F[5][c] = pyy; F[6][c] = pzz; F[7][c] = pxy; F[8][c] = pxz; F[9][c] = pyz;
+            F[11][c] = nrx; F[12][c] = nry; F[13][c] = nrz; F[14][c] = ind;
+            F[15][c] = fsx; F[16][c] = fsy; F[17][c] = fsz;
+        }
+        if constexpr (JET) {
+            if (z == 1 || z == nz - 2) {
+                const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+                float* b = p.bcp + (z == 1 ? 0u : (uint32_t)kPlaneVals * plane) + (uint32_t)x + (uint32_t)nx * (uint32_t)y;
+                const float v[kPlaneVals] = {rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, fsx, fsy, fsz};
+#pragma unroll
+                for (int k = 0; k < kPlaneVals; ++k) b[(uint32_t)k * plane] = v[k];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_jet_bc: nozzle inflow at z = 1 and convective outflow at z = nz-2,
+// written where the next step reads them.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {  // src/BoundaryConditions.cuh:172-181
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__device__ __forceinline__ float uniform01(uint32_t s) {  // :183-188
+#pragma clang fp contract(off)
+    return ((float)s + 0.5f) * 2.3283064365386963e-10f;
+}
+
+__device__ __forceinline__ uint32_t noise_base(const KParams& p, uint32_t salt, uint32_t x, uint32_t y) {
+    return (p.seed ^ salt) ^ x ^ (y * 0x85EBCA6Bu) ^ (p.t * 0xC2B2AE35u);
+}
+
+// Nozzle velocity at (x, y) for this step (src/BoundaryConditions.cuh:58-69).
+__device__ __forceinline__ void nozzle_u(const KParams& p, int x, int y, float& ux, float& uy, float& uz) {
+#pragma clang fp contract(off)
+    ux = 0.0f;
+    uy = 0.0f;
+    uz = p.u_inf;
+    if (p.noise == 0) {  // Gaussian on u_x, u_y (Box-Muller, :190-213)
+        const float sigma_u = 0.08f * p.u_inf;
+        const uint32_t salts[2] = {0xA341316Cu, 0xC8013EA4u};
+        float z2[2];
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t base = noise_base(p, salts[k], (uint32_t)x, (uint32_t)y);
+            float rrx = uniform01(hash32(base));
+            const float rry = uniform01(hash32(base ^ 0x68BC21EBu));
+            rrx = fmaxf(rrx, 1e-12f);
+            const float r = sqrtf(-2.0f * logf(rrx));
+            const float theta = (2.0f * 3.141592654f) * rry;
+            z2[k] = r * cosf(theta);
+        }
+        ux = sigma_u * z2[0];
+        uy = sigma_u * z2[1];
+    } else if (p.noise == 1) {  // uniform +-a on u_z
+        const float U = uniform01(hash32(noise_base(p, 0xA341316Cu, (uint32_t)x, (uint32_t)y)));
+        uz = p.u_inf * (1.0f + p.noise_amp * (2.0f * U - 1.0f));
+    }
+}
+
+__device__ __forceinline__ bool in_nozzle(const KParams& p, int x, int y) {
+#pragma clang fp contract(off)
+    const float dx = (float)x - p.cxg, dy = (float)y - p.cyg;
+    return dx * dx + dy * dy <= p.R2;
+}
+
+template <int Q, bool HALF, int NPAR>
+__global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
+#pragma clang fp contract(off)
+    const int x = 1 + blockIdx.x * kTX + (threadIdx.x & (kTX - 1));
+    const int y = 1 + blockIdx.y * kTY + (threadIdx.x >> 6);
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    if (x > nx - 2 || y > ny - 2) return;
+    const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t pi = (uint32_t)x + (uint32_t)nx * (uint32_t)y;
+    float* const* F = p.fld;
+
+    // ---------------- inflow, target cell (x, y, 1) ----------------
+    {
+        const float* b = p.bcp;
+        const float tux = b[1 * plane + pi], tuy = b[2 * plane + pi], tuz = b[3 * plane + pi];
+        const float pxx = b[4 * plane + pi], pyy = b[5 * plane + pi], pzz = b[6 * plane + pi];
+        const float pxy = b[7 * plane + pi], pxz = b[8 * plane + pi], pyz = b[9 * plane + pi];
+        const uint32_t c = pi + plane;  // z = 1
+        const Nbr nb = make_nbr<false>(x, y, 1, nx, ny, nz);
+        static_for<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (kCZ[q] == 1) {
+                const int xs = x - kCX[q], ys = y - kCY[q];
+                float val = 0.0f;  // frozen equilibrium f_eq(1, 0) - w = 0 off the nozzle
+                if (in_nozzle(p, xs, ys)) {
+                    float ux, uy, uz;
+                    nozzle_u(p, xs, ys, ux, uy, uz);
+                    const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+                    const float cu = cu_of<Q, q>(ux, uy, uz);
+                    val = f_eq<Q, q>(1.0f, uu, cu) + p.omco_zmin * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz);
+                }
+                int slot;
+                uint32_t i;
+                in_slot<q, NPAR>(nb, c, slot, i);
+                Pop<HALF>::st(p.f, p.N, slot, i, val);
+            }
+        });
+        float g5 = 0.0f;
+        if (in_nozzle(p, x, y)) {
+            float ux, uy, uz;
+            nozzle_u(p, x, y, ux, uy, uz);
+            g5 = wq<7>(5) * 1.0f * (1.0f + kAs2g * uz);
+            F[10][pi] = 1.0f;  // ghost phi at z = 0
+            if (p.save) {
+                F[0][pi] = 1.0f;
+                F[1][pi] = ux;
+                F[2][pi] = uy;
+                F[3][pi] = uz;
+            }
+        }
+        int slot;
+        uint32_t i;
+        in_slot<5, NPAR>(nb, c, slot, i);
+        Pop<HALF>::st(p.g, p.N, slot, i, HALF ? g5 - p.wphi0[5] : g5);
+    }
+
+    // ---------------- outflow, target cell (x, y, nz-2) ----------------
+    {
+        const float* b = p.bcp + (uint32_t)kPlaneVals * plane;
+        const float tux = b[1 * plane + pi], tuy = b[2 * plane + pi], tuz = b[3 * plane + pi];
+        const float pxx = b[4 * plane + pi], pyy = b[5 * plane + pi], pzz = b[6 * plane + pi];
+        const float pxy = b[7 * plane + pi], pxz = b[8 * plane + pi], pyz = b[9 * plane + pi];
+        const float tfx = b[10 * plane + pi], tfy = b[11 * plane + pi], tfz = b[12 * plane + pi];
+        const uint32_t zoff = plane * (uint32_t)(nz - 2);
+        const uint32_t c = pi + zoff;
+        const Nbr nb = make_nbr<false>(x, y, nz - 2, nx, ny, nz);
+        const float* phi_g = F[10];
+        const int xw3[3] = {(x == 1) ? nx - 2 : x - 1, x, (x == nx - 2) ? 1 : x + 1};
+        const int yw3[3] = {(y == 1) ? ny - 2 : y - 1, y, (y == ny - 2) ? 1 : y + 1};
+        static_for<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (kCZ[q] == -1) {
+                // source column (wrapped x - c_x, y - c_y) whose boundary
+                // values feed this target (src/BoundaryConditions.cuh:139-147)
+                const uint32_t si = (uint32_t)xw3[1 - kCX[q]] + (uint32_t)nx * (uint32_t)yw3[1 - kCY[q]];
+                const float rho_b = b[0 * plane + si];
+                const float ux = b[1 * plane + si], uy = b[2 * plane + si], uz = b[3 * plane + si];
+                const float phi_b = phi_g[si + zoff];
+                const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+                const float cu = cu_of<Q, q>(ux, uy, uz);
+                const float visc_local = (1.0f - phi_b) * p.visc_water + phi_b * p.visc_oil;
+                const float omega_zmax =
+                    (float)(1.0 / (0.5 + (double)kAs2 * (double)(visc_local * (1.0f + 100.0f))));
+                const float omco_zmax = 1.0f - omega_zmax;
+                const float val = f_eq<Q, q>(rho_b, uu, cu) +
+                                  omco_zmax * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz) +
+                                  0.5f * force_of<Q, q>(cu, ux, uy, uz, tfx, tfy, tfz);
+                int slot;
+                uint32_t i;
+                in_slot<q, NPAR>(nb, c, slot, i);
+                Pop<HALF>::st(p.f, p.N, slot, i, val);
+            }
+        });
+        const float phi_b = phi_g[c];
+        const float uz_b = b[3 * plane + pi];
+        const float g6 = wq<7>(6) * phi_b * (1.0f - kAs2g * uz_b);
+        int slot;
+        uint32_t i;
+        in_slot<6, NPAR>(nb, c, slot, i);
+        Pop<HALF>::st(p.g, p.N, slot, i, HALF ? g6 - p.wphi0[6] : g6);
+        // zero-gradient ghost plane z = nz-1
+        const uint32_t cg = c + plane;
+        F[10][cg] = phi_b;
+        if (p.save) {
+            F[0][cg] = b[0 * plane + pi];
+            F[1][cg] = b[1 * plane + pi];
+            F[2][cg] = b[2 * plane + pi];
+            F[3][cg] = uz_b;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <bool JET, bool HALF, int PAR>
+static hipError_t phase_sum_t(const KParams& p, hipStream_t s) {
+    dim3 grid((p.nx + kTX - 1) / kTX, (p.ny + kTY - 1) / kTY, p.nz - 2);
+    hipLaunchKernelGGL((k_phase_sum<JET, HALF, PAR>), grid, dim3(kThreads), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_phase_sum(const KParams& p, int Q, bool jet, bool half, int parity, hipStream_t s) {
+    (void)Q;
+    if (jet) {
+        if (half) return parity ? phase_sum_t<true, true, 1>(p, s) : phase_sum_t<true, true, 0>(p, s);
+        return parity ? phase_sum_t<true, false, 1>(p, s) : phase_sum_t<true, false, 0>(p, s);
+    }
+    if (half) return parity ? phase_sum_t<false, true, 1>(p, s) : phase_sum_t<false, true, 0>(p, s);
+    return parity ? phase_sum_t<false, false, 1>(p, s) : phase_sum_t<false, false, 0>(p, s);
+}
+
+template <int Q, bool JET, bool HALF, bool MLP>
+static hipError_t step_t(const KParams& p, int parity, hipStream_t s) {
+    const uint32_t nblk = (uint32_t)p.ntx * (uint32_t)p.nty * (uint32_t)p.ntz;
+    if (parity)
+        hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP>), dim3(nblk), dim3(kThreads), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP>), dim3(nblk), dim3(kThreads), 0, s, p);
+    return hipGetLastError();
+}
+
+template <int Q, bool JET, bool HALF>
+static hipError_t step_m(const KParams& p, int parity, bool mlp, hipStream_t s) {
+    if (mlp) {
+        if constexpr (kMlpKernels) return step_t<Q, JET, HALF, true>(p, parity, s);
+        return hipErrorNotSupported;
+    }
+    return step_t<Q, JET, HALF, false>(p, parity, s);
+}
+
+template <int Q>
+static hipError_t step_q(const KParams& p, bool jet, bool half, int parity, bool mlp, hipStream_t s) {
+    if (jet) return half ? step_m<Q, true, true>(p, parity, mlp, s) : step_m<Q, true, false>(p, parity, mlp, s);
+    return half ? step_m<Q, false, true>(p, parity, mlp, s) : step_m<Q, false, false>(p, parity, mlp, s);
+}
+
+hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, bool mlp, hipStream_t s) {
+    return Q == 27 ? step_q<27>(p, jet, half, parity, mlp, s) : step_q<19>(p, jet, half, parity, mlp, s);
+}
+
+template <int Q, bool HALF>
+static hipError_t jet_bc_t(const KParams& p, int npar, hipStream_t s) {
+    dim3 grid((p.nx - 2 + kTX - 1) / kTX, (p.ny - 2 + kTY - 1) / kTY, 1);
+    if (npar)
+        hipLaunchKernelGGL((k_jet_bc<Q, HALF, 1>), grid, dim3(kThreads), 0, s, p);
+    else
+        hipLaunchKernelGGL((k_jet_bc<Q, HALF, 0>), grid, dim3(kThreads), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hipStream_t s) {
+    if (Q == 27) return half ? jet_bc_t<27, true>(p, next_parity, s) : jet_bc_t<27, false>(p, next_parity, s);
+    return half ? jet_bc_t<19, true>(p, next_parity, s) : jet_bc_t<19, false>(p, next_parity, s);
+}
+
+}  // namespace pf

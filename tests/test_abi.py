@@ -1,0 +1,70 @@
+"""The C-ABI library loads and exports every declared symbol (no GPU work)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import phasefieldlbm_amd as pf
+from phasefieldlbm_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "phasefieldlbm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pf_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    lib = _abi.load()
+    syms = declared_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_abi.SIGNATURES), set(syms) ^ set(_abi.SIGNATURES)
+
+
+def test_config_struct_layout():
+    # the ctypes mirror has the C size: 28 four-byte members
+    assert ctypes.sizeof(_abi.PfConfig) == 4 * len(_abi.PfConfig._fields_)
+    text = open(os.path.join(ROOT, "include", "phasefieldlbm.h")).read()
+    body = text[text.index("typedef struct pf_config {"):text.index("} pf_config;")]
+    names = re.findall(r"\b(?:int32_t|uint32_t|float)\s+([a-z_0-9, ]+);", body)
+    flat = [n.strip() for grp in names for n in grp.split(",")]
+    assert flat == [f[0] for f in _abi.PfConfig._fields_]
+
+
+def test_reference_defaults():
+    jet = pf.Config.reference(pf.JET, pf.D3Q19)
+    assert (jet.nx, jet.ny, jet.nz, jet.diam) == (64, 64, 128, 10)
+    assert jet.reynolds_water == 5000 and jet.weber == 500 and abs(jet.u_inf - 0.05) < 1e-9
+    dr = pf.Config.reference(pf.DROPLET, pf.D3Q27)
+    assert (dr.nx, dr.radius, dr.axis_x) == (128, 35.0, 2.0)
+    assert abs(dr.sigma - 0.03) < 1e-9 and abs(dr.tau - 0.55) < 1e-7
+
+
+@pytest.mark.parametrize("field,value", [("velocity_set", 15), ("flow_case", 7), ("storage", 9), ("nx", 2),
+                                         ("noise", 5), ("interface_width", 0.0)])
+def test_bad_config_rejected(field, value):
+    cfg = pf.presets.static_droplet(n=16)
+    setattr(cfg, field, value)
+    with pytest.raises(pf.PfError) as e:
+        pf.Simulation(cfg)
+    assert e.value.status == -1
+
+
+def test_status_strings():
+    lib = _abi.load()
+    for code in range(0, -8, -1):
+        assert lib.pf_status_string(code).decode() == _abi.STATUS[code]
+
+
+def test_no_device_reports_device_error():
+    from tests.conftest import HAVE_GPU
+    if HAVE_GPU:
+        pytest.skip("a GPU is present")
+    with pytest.raises(pf.PfError) as e:
+        pf.Simulation(pf.presets.static_droplet(n=16))
+    assert e.value.status == -2

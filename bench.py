@@ -90,7 +90,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP16 form and Laplace probes")
     args = ap.parse_args()
@@ -116,11 +116,16 @@ def main():
     log(f"[bench] fp32: {mlups:.0f} MLUPS, {ms:.3f} ms/step, kernel {step_ms:.3f} ms, "
         f"phase {tm['phase_sum_ms'] / args.steps:.3f} ms, bc {tm['bc_ms'] / args.steps:.3f} ms")
 
+    # HBM bytes per k_step launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE summary (a "traffic_bytes_per_launch:" line), if present.
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_k_step_jet_d3q27_256x256x1024_fp32.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_k_step_jet_d3q27_256x256x1024_fp32.md")
     if os.path.exists(tpath):
-        with open(tpath) as fh:
-            traffic = json.load(fh).get("bytes_per_launch")
+        for line in open(tpath):
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                traffic = int(float(line.split(":", 1)[1]))
+    copy_gbs = pf.copy_bandwidth(0, 2 << 30, 10)
+    log(f"[bench] streaming-copy bandwidth {copy_gbs:.0f} GB/s")
 
     out = {
         "metric": "MLUPS (D3Q27 round jet 256x256x1024, FP32 populations)",
@@ -141,6 +146,8 @@ def main():
                    "storage": "fp32", "parallelism": "single GPU"},
         "interior_mlups": round(inner * args.steps / wall / 1e6, 1),
         "step_roofline_frac": round(mlups * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9), 4),
+        "measured_copy_gbs": round(copy_gbs, 1),
+        "step_frac_of_copy": round(mlups * 1e6 * b_alg / (copy_gbs * 1e9), 4),
         "bytes_per_update": b_alg,
         "roofline": {"bound": "hbm", "kernel": "k_step", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

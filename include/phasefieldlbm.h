@@ -48,6 +48,13 @@ enum pf_noise { PF_NOISE_GAUSSIAN_XY = 0, PF_NOISE_UNIFORM_Z = 1, PF_NOISE_NONE 
  * or the learned MLP at interface cells with the FD value as fallback. */
 enum pf_closure { PF_CLOSURE_FD = 0, PF_CLOSURE_MLP = 1 };
 
+/* Collision evaluation.  PAIR evaluates each opposite pair of directions
+ * once (even and odd parts in c) with closed-form equilibrium and forcing
+ * moments; LITERAL evaluates the reference's per-direction formulas
+ * (src/lbm.cu:33-207) term by term.  Both are the same scheme; they differ by
+ * floating-point rounding only. */
+enum pf_kernel { PF_KERNEL_PAIR = 0, PF_KERNEL_LITERAL = 1 };
+
 /* Field identifiers; 0..17 follow host::FieldID (src/fileIO/fields.cuh:33-68). */
 enum pf_field {
     PF_FIELD_RHO = 0, PF_FIELD_UX, PF_FIELD_UY, PF_FIELD_UZ,
@@ -100,11 +107,13 @@ typedef struct pf_config {
     /* phase field */
     float interface_width;
     float tau_g;           /* only sets the sharpening gamma, as in the reference */
-    float phi0;            /* FP16 storage shift of g (g - w*phi0) */
+    float phi0;            /* FP16 storage shift of g (g - w*phi0); default 0 */
     /* execution */
     int32_t device;        /* HIP device index; <0 reads GPU_INDEX like setDeviceFromEnv */
     int32_t zchunk;        /* z cells marched per workgroup; 0 = automatic */
     float mlp_lo, mlp_hi;  /* MLP applies where mlp_lo < phi < mlp_hi */
+    int32_t kernel;        /* pf_kernel */
+    int32_t slot_pad;      /* cells of padding between population slots; <0 = automatic */
 } pf_config;
 
 typedef struct pf_ctx pf_ctx;
@@ -169,6 +178,12 @@ int pf_set_closure(pf_ctx *ctx, const float *W1, const float *b1, const float *W
  * out_ms[2] = fused collide-stream kernel total, out_ms[3] = boundary total.
  * Per-kernel times are measured in a second, separately-evented pass. */
 int pf_time_steps(pf_ctx *ctx, uint64_t t0, uint32_t nsteps, float out_ms[4]);
+
+/* Achievable HBM bandwidth of `device`: a streaming copy of `bytes` bytes
+ * repeated `iters` times, timed with HIP events; *gbs = (read + write bytes)
+ * / time.  No counterpart in the reference; used to state the measured peak
+ * beside the roofline and to calibrate memory-traffic counters. */
+int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double *gbs);
 
 /* Bytes of device memory held by the context. */
 size_t pf_device_bytes(const pf_ctx *ctx);

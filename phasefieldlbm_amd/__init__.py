@@ -17,16 +17,25 @@ from typing import Optional
 import numpy as np
 
 from . import _abi
-from ._abi import (CLOSURE_FD, CLOSURE_MLP, D3Q19, D3Q27, DROPLET, FIELDS, JET, NOISE_GAUSSIAN_XY,
-                   NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, STORE_FP32)
+from ._abi import (CLOSURE_FD, CLOSURE_MLP, D3Q19, D3Q27, DROPLET, FIELDS, JET, KERNEL_LITERAL, KERNEL_PAIR,
+                   NOISE_GAUSSIAN_XY, NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, STORE_FP32)
 
 __all__ = ["Config", "Simulation", "PfError", "presets", "FIELDS", "D3Q19", "D3Q27", "JET", "DROPLET",
            "STORE_FP32", "STORE_FP16", "NOISE_GAUSSIAN_XY", "NOISE_UNIFORM_Z", "NOISE_NONE",
-           "CLOSURE_FD", "CLOSURE_MLP", "library_path"]
+           "CLOSURE_FD", "CLOSURE_MLP", "KERNEL_PAIR", "KERNEL_LITERAL", "library_path", "copy_bandwidth"]
 
 
 def library_path() -> str:
     return _abi.LIB_PATH
+
+
+def copy_bandwidth(device: int = 0, nbytes: int = 2 << 30, iters: int = 20) -> float:
+    """Measured HBM bandwidth (GB/s, read + write) of a streaming copy."""
+    g = ctypes.c_double()
+    rc = _abi.load().pf_copy_bandwidth(device, nbytes, iters, ctypes.byref(g))
+    if rc:
+        raise PfError(rc, "pf_copy_bandwidth")
+    return g.value
 
 
 class PfError(RuntimeError):
@@ -66,6 +75,8 @@ class Config:
     zchunk: int = 0
     mlp_lo: float = 0.05
     mlp_hi: float = 0.95
+    kernel: int = KERNEL_PAIR
+    slot_pad: int = -1
 
     @classmethod
     def reference(cls, flow_case: int, velocity_set: int) -> "Config":

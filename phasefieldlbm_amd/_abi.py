@@ -18,6 +18,7 @@ JET, DROPLET = 0, 1
 STORE_FP32, STORE_FP16 = 0, 1
 NOISE_GAUSSIAN_XY, NOISE_UNIFORM_Z, NOISE_NONE = 0, 1, 2
 CLOSURE_FD, CLOSURE_MLP = 0, 1
+KERNEL_PAIR, KERNEL_LITERAL = 0, 1
 STEP_SAVE_LAST = 1
 
 # host::FieldID order (src/fileIO/fields.cuh:33-68)
@@ -56,6 +57,8 @@ class PfConfig(ctypes.Structure):
         ("device", ctypes.c_int32),
         ("zchunk", ctypes.c_int32),
         ("mlp_lo", ctypes.c_float), ("mlp_hi", ctypes.c_float),
+        ("kernel", ctypes.c_int32),
+        ("slot_pad", ctypes.c_int32),
     ]
 
 
@@ -76,6 +79,8 @@ SIGNATURES = {
     "pf_set_closure": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6),
     "pf_time_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p]),
     "pf_device_bytes": (ctypes.c_size_t, [ctypes.c_void_p]),
+    "pf_copy_bandwidth": (ctypes.c_int, [ctypes.c_int32, ctypes.c_size_t, ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_double)]),
     "pf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "pf_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -57,13 +57,13 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
     const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
     static_for<Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        st_pop<HALF>(p.f, p.N, q, i, f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
+        st_pop<HALF>(p.f, p.sN, q, i, f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
     });
     static_for<7>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         float v = g_post<q>(phi, ux, uy, uz, 0.0f, 0.0f, 0.0f, 0.0f);
         if constexpr (HALF) v -= p.wphi0[q];
-        st_pop<HALF>(p.g, p.N, q, i, v);
+        st_pop<HALF>(p.g, p.sN, q, i, v);
     });
 }
 
@@ -97,9 +97,9 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
                       (uint32_t)nx * ((uint32_t)ys[1 - kCY[q]] + (uint32_t)ny * (uint32_t)zs[1 - kCZ[q]]);
             }
             if (which == 0) {
-                v = ld_pop<HALF>(p.f, p.N, slot, src);
+                v = ld_pop<HALF>(p.f, p.sN, slot, src);
             } else {
-                v = ld_pop<HALF>(p.g, p.N, slot, src);
+                v = ld_pop<HALF>(p.g, p.sN, slot, src);
                 if constexpr (HALF) v += p.wphi0[q];
             }
         }
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials)
                 src = (uint32_t)xs[1 - kCX[q]] +
                       (uint32_t)nx * ((uint32_t)ys[1 - kCY[q]] + (uint32_t)ny * (uint32_t)zs[1 - kCZ[q]]);
             }
-            float gv = ld_pop<HALF>(p.g, p.N, slot, src);
+            float gv = ld_pop<HALF>(p.g, p.sN, slot, src);
             if constexpr (HALF) gv += p.wphi0[q];
             phi += (double)gv;
         }
@@ -212,7 +212,29 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* partials,
     }
 }
 
+// Streaming copy used to measure the achievable HBM bandwidth (and to
+// calibrate memory-traffic counters against a known byte count).
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+    // four independent 16-byte loads in flight per thread before the stores
+    const size_t stride = (size_t)gridDim.x * 1024u;
+    size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x;
+    for (; i + 768 < n4; i += stride) {
+        const float4 a = src[i], b = src[i + 256], c = src[i + 512], d = src[i + 768];
+        dst[i] = a;
+        dst[i + 256] = b;
+        dst[i + 512] = c;
+        dst[i + 768] = d;
+    }
+    for (; i < n4; i += 256) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
+hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s) {
+    const size_t blocks = (n4 + 1023) / 1024;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, src, dst, n4);
+    return hipGetLastError();
+}
+
 hipError_t launch_init_fields(const KParams& p, bool jet, float, float, float, float, float, hipStream_t s) {
     hipLaunchKernelGGL(k_init_fields, dim3((p.N + 255) / 256), dim3(256), 0, s, p, jet ? 1 : 0);
     return hipGetLastError();

@@ -4,7 +4,9 @@
 // HBM layout (one context):
 //   f   : Q slots x N cells, FP32 or binary16, slot-major (SoA), the
 //         reference's f[q*N + idx] (src/functions/deviceFunctions.cuh:33-48)
-//         but addressed in place with the A-A pattern (see step kernels)
+//         with the slot stride padded to sN >= N (a stride of a large power
+//         of two puts every slot of a cell on the same HBM channel), and
+//         addressed in place with the A-A pattern (see step kernels)
 //   g   : 7 slots x N cells, same storage type as f
 //   fld : 18 FP32 macroscopic fields [N] in host::FieldID order; phi is
 //         live every step, the others are written on save steps only
@@ -20,9 +22,22 @@
 
 namespace pf {
 
-constexpr int kTX = 64;  // tile width in x (one wavefront)
-constexpr int kTY = 4;   // tile rows in y (4 wavefronts per workgroup)
+// Workgroup tile of the timestep kernels: kTX cells in x (a multiple of the
+// 64-lane wavefront, a power of two) by kTY rows in y.  128 x 4 measured best
+// on MI355X for the D3Q27 jet: a wider x tile halves the cache lines that two
+// tiles share in the in-place odd step (each x-shifted population of a tile
+// row touches one line of the neighbouring tile).
+#ifndef PF_TILE_X
+#define PF_TILE_X 128
+#endif
+#ifndef PF_TILE_Y
+#define PF_TILE_Y 4
+#endif
+constexpr int kTX = PF_TILE_X;
+constexpr int kTY = PF_TILE_Y;
 constexpr int kThreads = kTX * kTY;
+static_assert((kTX & (kTX - 1)) == 0 && kTX >= 64, "tile width must be a power of two >= 64");
+static_assert(kThreads <= 1024, "workgroup too large");
 
 constexpr int kNumFields = 18;
 constexpr int kPlaneVals = 13;  // rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, fsx, fsy, fsz
@@ -37,7 +52,8 @@ struct KParams {
     float* fld[kNumFields];
     float* bcp;
     int nx, ny, nz;
-    uint32_t N;
+    uint32_t N;   // cells
+    uint32_t sN;  // population slot stride in elements (N plus padding)
     int zchunk;
     int ntx, nty, ntz;  // tile counts
     int save;           // write every macroscopic field this step
@@ -62,7 +78,9 @@ struct KParams {
 
 // Launchers; all return hipError_t of the launch.
 hipError_t launch_phase_sum(const KParams& p, int Q, bool jet, bool half, int parity, hipStream_t s);
-hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, bool mlp, hipStream_t s);
+// variant: 0 pair-symmetric collision, 1 literal per-direction evaluation,
+// 2 pair-symmetric with the learned curvature closure
+hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, int variant, hipStream_t s);
 hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hipStream_t s);
 hipError_t launch_init_fields(const KParams& p, bool jet, float radius, float ax, float ay, float az, float W,
                               hipStream_t s);
@@ -73,5 +91,6 @@ hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, h
 hipError_t launch_phase_mass(const KParams& p, bool jet, bool half, int parity, double* partials, int nblocks,
                              hipStream_t s);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s);
+hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
 
 }  // namespace pf

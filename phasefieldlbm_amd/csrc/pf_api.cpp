@@ -21,6 +21,7 @@ struct pf_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
     size_t N = 0;
+    size_t sN = 0;  // population slot stride (elements)
     int Q = 19;
     bool jet = true, half = false;
     void* f = nullptr;
@@ -117,11 +118,16 @@ void fill_constants(pf_ctx* c) {
     k.mlp_hi = g.mlp_hi;
 }
 
+// z cells marched per workgroup.  Measured on MI355X (D3Q27 jet
+// 256x256x1024): FP32 is fastest with ~16k workgroups (z-chunk 8), the
+// binary16 form with ~4k (z-chunk 32); short chunks re-load more halo planes,
+// long ones leave the last wave of workgroups unbalanced.
 int choose_zchunk(const pf_config& g) {
     if (g.zchunk > 0) return g.zchunk;
     const int ntx = (g.nx + pf::kTX - 1) / pf::kTX, nty = (g.ny + pf::kTY - 1) / pf::kTY;
     const long cols = (long)ntx * nty;
-    long zc = (long)(g.nz - 2) * cols / 4096;
+    const long target = g.storage == PF_STORE_FP16 ? 4096 : 16384;
+    long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;
     return (int)zc;
@@ -165,6 +171,7 @@ int pf_config_default(pf_config* cfg, int32_t flow_case, int32_t velocity_set) {
     cfg->noise_amp = 0.05f;
     cfg->mlp_lo = 0.05f;
     cfg->mlp_hi = 0.95f;
+    cfg->slot_pad = -1;
     cfg->axis_x = 2.0f;  // src/initialConditions.cu:74 halves the x offset
     cfg->axis_y = 1.0f;
     cfg->axis_z = 1.0f;
@@ -201,6 +208,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     if (g.flow_case == PF_DROPLET && !(g.tau > 0.5f)) return PF_ERR_CONFIG;
     if (!(g.interface_width > 0) || g.axis_x == 0 || g.axis_y == 0 || g.axis_z == 0) return PF_ERR_CONFIG;
     if (g.noise < 0 || g.noise > 2) return PF_ERR_CONFIG;
+    if (g.kernel != PF_KERNEL_PAIR && g.kernel != PF_KERNEL_LITERAL) return PF_ERR_CONFIG;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
     if (N >= (size_t)1 << 30) return PF_ERR_CONFIG;  // byte offsets of one slot stay below 2^32
 
@@ -211,6 +219,15 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     c->Q = g.velocity_set;
     c->jet = g.flow_case == PF_JET;
     c->half = g.storage == PF_STORE_FP16;
+    // Slot stride: N rounded up to 256 cells plus `slot_pad` cells (default
+    // 1 KiB-worth of cells times an odd factor) so that successive slots of
+    // the same cell start at different HBM channel offsets.
+    const long pad = g.slot_pad >= 0 ? g.slot_pad : (c->half ? 1536 : 768);
+    c->sN = ((N + 255) / 256) * 256 + (size_t)pad;
+    if ((c->sN * 27 * (c->half ? 2 : 4)) >> 36) {  // keep every byte offset representable
+        delete c;
+        return PF_ERR_CONFIG;
+    }
 
     // device selection (src/functions/hostFunctions.cuh:117-150)
     int dev = g.device;
@@ -245,7 +262,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
         c->bytes += bytes;
         return true;
     };
-    bool ok = alloc(&c->f, (size_t)c->Q * N * pop_bytes(c)) && alloc(&c->g, (size_t)7 * N * pop_bytes(c));
+    bool ok = alloc(&c->f, (size_t)c->Q * c->sN * pop_bytes(c)) && alloc(&c->g, (size_t)7 * c->sN * pop_bytes(c));
     for (int i = 0; ok && i < pf::kNumFields; ++i) ok = alloc((void**)&c->fld[i], N * sizeof(float));
     const size_t plane = (size_t)g.nx * g.ny;
     ok = ok && alloc((void**)&c->bcp, 2 * pf::kPlaneVals * plane * sizeof(float));
@@ -266,6 +283,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     k.ny = g.ny;
     k.nz = g.nz;
     k.N = (uint32_t)N;
+    k.sN = (uint32_t)c->sN;
     k.zchunk = choose_zchunk(g);
     k.ntx = (g.nx + pf::kTX - 1) / pf::kTX;
     k.nty = (g.ny + pf::kTY - 1) / pf::kTY;
@@ -311,8 +329,8 @@ int pf_init(pf_ctx* c) {
     const pf_config& g = c->cfg;
     for (int i = 0; i < pf::kNumFields; ++i)
         PF_HIP(c, hipMemsetAsync(c->fld[i], 0, c->N * sizeof(float), c->stream));
-    PF_HIP(c, hipMemsetAsync(c->f, 0, (size_t)c->Q * c->N * pop_bytes(c), c->stream));
-    PF_HIP(c, hipMemsetAsync(c->g, 0, (size_t)7 * c->N * pop_bytes(c), c->stream));
+    PF_HIP(c, hipMemsetAsync(c->f, 0, (size_t)c->Q * c->sN * pop_bytes(c), c->stream));
+    PF_HIP(c, hipMemsetAsync(c->g, 0, (size_t)7 * c->sN * pop_bytes(c), c->stream));
     PF_HIP(c, pf::launch_init_fields(c->kp, c->jet, g.radius, g.axis_x, g.axis_y, g.axis_z, g.interface_width,
                                      c->stream));
     if (!c->jet) {
@@ -356,11 +374,11 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
     for (uint32_t s = 0; s < nsteps; ++s) {
         c->kp.t = (uint32_t)(t0 + s);
         c->kp.save = ((flags & PF_STEP_SAVE_LAST) && s + 1 == nsteps) ? 1 : 0;
-        const bool mlp = c->cfg.closure == PF_CLOSURE_MLP;
+        const int variant = c->cfg.closure == PF_CLOSURE_MLP ? 2 : c->cfg.kernel;
         if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
         PF_HIP(c, pf::launch_phase_sum(c->kp, c->Q, c->jet, c->half, c->parity, c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[1], c->stream));
-        PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, mlp, c->stream));
+        PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[2], c->stream));
         if (c->jet) PF_HIP(c, pf::launch_jet_bc(c->kp, c->Q, c->half, c->parity ^ 1, c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[3], c->stream));
@@ -499,6 +517,42 @@ int pf_time_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, float out_ms[4]) {
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return PF_OK;
+}
+
+int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double* gbs) {
+    if (!gbs || bytes < 1024 || iters == 0) return PF_ERR_CONFIG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return PF_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return PF_ERR_DEVICE;
+    const size_t n4 = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, n4 * 16) != hipSuccess) return PF_ERR_OOM;
+    if (hipMalloc((void**)&b, n4 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return PF_ERR_OOM;
+    }
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    int rc = PF_OK;
+    if (hipStreamCreate(&s) != hipSuccess) rc = PF_ERR_DEVICE;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = PF_ERR_DEVICE;
+    if (!rc) {
+        (void)hipMemsetAsync(a, 0, n4 * 16, s);
+        (void)pf::launch_copy(a, b, n4, s);  // warm-up
+        (void)hipEventRecord(e0, s);
+        for (uint32_t i = 0; i < iters; ++i) (void)pf::launch_copy(a, b, n4, s);
+        (void)hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = PF_ERR_LAUNCH;
+        float ms = 0.0f;
+        if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PF_ERR_LAUNCH;
+        if (!rc) *gbs = 2.0 * (double)(n4 * 16) * iters / (ms * 1e-3) / 1e9;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
 }
 
 }  // extern "C"

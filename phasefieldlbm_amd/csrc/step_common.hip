@@ -1,0 +1,28 @@
+// Velocity-set independent launches: phase sum and dispatch to the per-Q units.
+#include "step_kernels.hpp"
+
+namespace pf {
+
+hipError_t launch_step_q19(const KParams& p, bool jet, bool half, int parity, int variant, hipStream_t s);
+hipError_t launch_step_q27(const KParams& p, bool jet, bool half, int parity, int variant, hipStream_t s);
+hipError_t launch_jet_bc_q19(const KParams& p, bool half, int next_parity, hipStream_t s);
+hipError_t launch_jet_bc_q27(const KParams& p, bool half, int next_parity, hipStream_t s);
+
+hipError_t launch_phase_sum(const KParams& p, int, bool jet, bool half, int parity, hipStream_t s) {
+    if (jet) {
+        if (half) return parity ? phase_sum_t<true, true, 1>(p, s) : phase_sum_t<true, true, 0>(p, s);
+        return parity ? phase_sum_t<true, false, 1>(p, s) : phase_sum_t<true, false, 0>(p, s);
+    }
+    if (half) return parity ? phase_sum_t<false, true, 1>(p, s) : phase_sum_t<false, true, 0>(p, s);
+    return parity ? phase_sum_t<false, false, 1>(p, s) : phase_sum_t<false, false, 0>(p, s);
+}
+
+hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, int variant, hipStream_t s) {
+    return Q == 27 ? launch_step_q27(p, jet, half, parity, variant, s) : launch_step_q19(p, jet, half, parity, variant, s);
+}
+
+hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hipStream_t s) {
+    return Q == 27 ? launch_jet_bc_q27(p, half, next_parity, s) : launch_jet_bc_q19(p, half, next_parity, s);
+}
+
+}  // namespace pf

@@ -23,6 +23,8 @@
 // halo live in LDS, so phi is read from HBM about once per cell and the
 // normals never leave the chip (the reference stores normx/y/z, ind and the
 // force in HBM between three kernels).
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include "kparams.hpp"
@@ -41,16 +43,33 @@ constexpr int kNY1 = kTY + 2;
 // ---------------------------------------------------------------------------
 template <bool HALF>
 struct Pop;
+// Population stores are consumed only by the next step, so they carry the
+// non-temporal (streaming) hint (measured ~7% faster on MI355X).  Loads keep
+// the default policy: in the odd step neighbouring tiles re-read the lines at
+// their common edge, and streaming loads measured slower.
+template <class T>
+__device__ __forceinline__ T pop_load(const T* p) {
+    return *p;
+}
+template <class T>
+__device__ __forceinline__ void pop_store(T* p, T v) {
+#ifdef PF_CACHED_STORE
+    *p = v;
+#else
+    __builtin_nontemporal_store(v, p);
+#endif
+}
+
 template <>
 struct Pop<false> {
     using T = float;
     static __device__ __forceinline__ float ld(const void* base, uint32_t N, int slot, uint32_t i) {
         const char* b = reinterpret_cast<const char*>(base) + (size_t)slot * N * 4u;
-        return *reinterpret_cast<const float*>(b + (i << 2));
+        return pop_load(reinterpret_cast<const float*>(b + (i << 2)));
     }
     static __device__ __forceinline__ void st(void* base, uint32_t N, int slot, uint32_t i, float v) {
         char* b = reinterpret_cast<char*>(base) + (size_t)slot * N * 4u;
-        *reinterpret_cast<float*>(b + (i << 2)) = v;
+        pop_store(reinterpret_cast<float*>(b + (i << 2)), v);
     }
 };
 template <>
@@ -58,13 +77,23 @@ struct Pop<true> {
     using T = _Float16;
     static __device__ __forceinline__ float ld(const void* base, uint32_t N, int slot, uint32_t i) {
         const char* b = reinterpret_cast<const char*>(base) + (size_t)slot * N * 2u;
-        return (float)*reinterpret_cast<const _Float16*>(b + (i << 1));
+        return (float)pop_load(reinterpret_cast<const _Float16*>(b + (i << 1)));
     }
     static __device__ __forceinline__ void st(void* base, uint32_t N, int slot, uint32_t i, float v) {
         char* b = reinterpret_cast<char*>(base) + (size_t)slot * N * 2u;
-        *reinterpret_cast<_Float16*>(b + (i << 1)) = (_Float16)v;
+        pop_store(reinterpret_cast<_Float16*>(b + (i << 1)), (_Float16)v);
     }
 };
+
+// Reciprocal: IEEE division by default; the hardware reciprocal (1 ulp)
+// when PF_FAST_RCP is defined.
+__device__ __forceinline__ float recip(float x) {
+#ifdef PF_FAST_RCP
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
 
 // Neighbour offsets of one cell: xw[dx+1] + row[dy+1][dz+1].
 struct Nbr {
@@ -112,29 +141,41 @@ __device__ __forceinline__ void in_slot(const Nbr& n, uint32_t c, int& slot, uin
 }
 
 // ---------------------------------------------------------------------------
-// k_phase_sum: phi = sum_q g_q over interior cells
+// k_phase_sum: phi = sum_q g_q over interior cells (src/phaseField.cuh:31-52).
+// Each thread sums kPsZ cells stacked in z so that 7*kPsZ loads are in flight.
 // ---------------------------------------------------------------------------
+constexpr int kPsZ = 4;
+
 template <bool JET, bool HALF, int PAR>
 __global__ __launch_bounds__(kThreads) void k_phase_sum(KParams p) {
     const int x = blockIdx.x * kTX + (threadIdx.x & (kTX - 1));
-    const int y = blockIdx.y * kTY + (threadIdx.x >> 6);
-    const int z = blockIdx.z + 1;
+    const int y = blockIdx.y * kTY + (int)(threadIdx.x / kTX);
     if (x < 1 || x > p.nx - 2 || y < 1 || y > p.ny - 2) return;
-    const uint32_t c = (uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z);
-    const Nbr n = make_nbr<!JET>(x, y, z, p.nx, p.ny, p.nz);
-    float g[7];
-    static_for<7>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        int slot;
-        uint32_t i;
-        in_slot<q, PAR>(n, c, slot, i);
-        g[q] = Pop<HALF>::ld(p.g, p.N, slot, i);
-        if constexpr (HALF) g[q] += p.wphi0[q];
-    });
-    float phi = 0.0f;
+    const int z0 = 1 + blockIdx.z * kPsZ;
+    float g[kPsZ][7];
 #pragma unroll
-    for (int q = 0; q < 7; ++q) phi += g[q];
-    p.fld[10][c] = phi;
+    for (int k = 0; k < kPsZ; ++k) {
+        const int z = min(z0 + k, p.nz - 2);  // clamped duplicates are not stored
+        const uint32_t c = (uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z);
+        const Nbr n = make_nbr<!JET>(x, y, z, p.nx, p.ny, p.nz);
+        static_for<7>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            int slot;
+            uint32_t i;
+            in_slot<q, PAR>(n, c, slot, i);
+            g[k][q] = Pop<HALF>::ld(p.g, p.sN, slot, i);
+            if constexpr (HALF) g[k][q] += p.wphi0[q];
+        });
+    }
+#pragma unroll
+    for (int k = 0; k < kPsZ; ++k) {
+        const int z = z0 + k;
+        if (z > p.nz - 2) break;
+        float phi = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) phi += g[k][q];
+        p.fld[10][(uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z)] = phi;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -143,19 +184,105 @@ __global__ __launch_bounds__(kThreads) void k_phase_sum(KParams p) {
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nblk) {
     // Workgroups are dealt round-robin over the 8 XCDs; give each XCD a
     // contiguous range of tiles so x-adjacent tiles share its L2.
+#ifdef PF_NO_XCD_REMAP
+    (void)nblk;
+    return b;
+#endif
     const uint32_t q = nblk >> 3, r = nblk & 7u;
     const uint32_t k = b & 7u;
     return k * q + min(k, r) + (b >> 3);
 }
 
-template <int Q, bool JET, bool HALF, int PAR, bool MLP>
+// c.v for a compile-time direction (zero components skipped, signs folded).
+template <int cx, int cy, int cz>
+__device__ __forceinline__ float cdot(float x, float y, float z) {
+    float s = 0.0f;
+    bool any = false;
+    if constexpr (cx != 0) {
+        s = cx > 0 ? x : -x;
+        any = true;
+    }
+    if constexpr (cy != 0) {
+        s = any ? (cy > 0 ? s + y : s - y) : (cy > 0 ? y : -y);
+        any = true;
+    }
+    if constexpr (cz != 0) s = any ? (cz > 0 ? s + z : s - z) : (cz > 0 ? z : -z);
+    return s;
+}
+
+// Per-cell quantities shared by all pairs of the regularized collision.
+struct Coll {
+    float B0;               // rho (1 - 1.5|u|^2) - 1 - 1.5 u.F : even part common to all directions
+    float R3, R45, V, O45;  // 3 rho, 4.5 rho, 1 - 1.5|u|^2, 4.5 (1 - omega)
+    float Pxx, Pyy, Pzz, Sxy, Sxz, Syz, S3, Txy, Txz, Tyz;  // stress sums, T = 2 P_ab
+    float X1, X2, X3, X4, X5, X6, X7s;  // third-order Hermite blocks (D3Q27), X7s = 6 X7
+    float Bx, By, Bz;                    // X3 + X5, X1 + X6, X2 + X4
+};
+
+// out(+c) and out(-c) of one opposite pair (c = direction of the odd member):
+//   even = w (B0 + 4.5 rho t^2 + 4.5 t s + 4.5 (1-omega) c.P.c)
+//   odd  = w (3 rho t [1 + 1.5 t^2 - 1.5|u|^2]_{D3Q27} + 1.5 s + 4.5 (1-omega) N3_{D3Q27})
+// with t = c.u and s = c.F.  This is f_eq + (1-omega) f_neq(P) + F_q/2 of the
+// reference (D3Q19.cuh:143-187, D3Q27.cuh:152-206, lbm.cu:206) split into
+// its even and odd parts in c, each evaluated once per pair.
+template <int Q, int cx, int cy, int cz>
+__device__ __forceinline__ void pair_out(const Coll& k, float ux, float uy, float uz, float fx, float fy, float fz,
+                                         float& out_p, float& out_m) {
+    constexpr float w = wk<Q>((cx != 0) + (cy != 0) + (cz != 0));
+    constexpr int nz_ = (cx != 0) + (cy != 0) + (cz != 0);
+    const float t = cdot<cx, cy, cz>(ux, uy, uz);
+    const float s = cdot<cx, cy, cz>(fx, fy, fz);
+    const float t2 = t * t;
+    float cpc;
+    if constexpr (nz_ == 1) {
+        cpc = cx != 0 ? k.Pxx : (cy != 0 ? k.Pyy : k.Pzz);
+    } else if constexpr (nz_ == 2) {
+        const float d = (cz == 0) ? k.Sxy : ((cy == 0) ? k.Sxz : k.Syz);
+        const float o = (cz == 0) ? k.Txy : ((cy == 0) ? k.Txz : k.Tyz);
+        cpc = ((cz == 0 ? cx * cy : (cy == 0 ? cx * cz : cy * cz)) > 0) ? d + o : d - o;
+    } else {
+        float o = cx * cy > 0 ? k.Txy : -k.Txy;
+        o = cx * cz > 0 ? o + k.Txz : o - k.Txz;
+        o = cy * cz > 0 ? o + k.Tyz : o - k.Tyz;
+        cpc = k.S3 + o;
+    }
+    const float even = w * (k.B0 + k.R45 * t2 + 4.5f * t * s + k.O45 * cpc);
+    float odd;
+    if constexpr (Q == 19) {
+        odd = w * (k.R3 * t + 1.5f * s);
+    } else {
+        // third-order block sum_i K_i X_i + 6 cx cy cz X7, K_i = 3 k_i of D3Q27.cuh:180-189
+        float nb;
+        if constexpr (nz_ == 3) {
+            nb = 2.0f * cdot<cx, cy, cz>(k.Bx, k.By, k.Bz);
+            nb = (cx * cy * cz > 0) ? nb + k.X7s : nb - k.X7s;
+        } else {
+            constexpr int K1 = cy * (3 * cx * cx - 1), K2 = cz * (3 * cx * cx - 1);
+            constexpr int K3 = cx * (3 * cy * cy - 1), K4 = cz * (3 * cy * cy - 1);
+            constexpr int K5 = cx * (3 * cz * cz - 1), K6 = cy * (3 * cz * cz - 1);
+            nb = 0.0f;
+            if constexpr (K1 != 0) nb = nb + (float)K1 * k.X1;
+            if constexpr (K2 != 0) nb = nb + (float)K2 * k.X2;
+            if constexpr (K3 != 0) nb = nb + (float)K3 * k.X3;
+            if constexpr (K4 != 0) nb = nb + (float)K4 * k.X4;
+            if constexpr (K5 != 0) nb = nb + (float)K5 * k.X5;
+            if constexpr (K6 != 0) nb = nb + (float)K6 * k.X6;
+        }
+        odd = w * (k.R3 * t * (k.V + 1.5f * t2) + 1.5f * s + k.O45 * nb);
+    }
+    out_p = even + odd;
+    out_m = even - odd;
+}
+
+template <int Q, bool JET, bool HALF, int PAR, bool MLP, bool LIT, bool SAVE>
 __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     __shared__ float s_phi[5][kHY * kHX];
     __shared__ float s_n[3][4][kNY1 * kNX1];
     __shared__ MlpShared<MLP> s_mlp;
+    constexpr int NP = (Q - 1) / 2;
 
     const int tid = threadIdx.x;
-    const int tx = tid & (kTX - 1), ty = tid >> 6;
+    const int tx = tid & (kTX - 1), ty = tid / kTX;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
     const int bx = (int)(lb % (uint32_t)p.ntx);
@@ -205,7 +332,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 const float gy_ = kAs2 * iso_sum<Q, 1>(val);
                 const float gz_ = kAs2 * iso_sum<Q, 2>(val);
                 ind = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_);
-                const float inv = 1.0f / (ind + 1e-9f);
+                const float inv = recip(ind + 1e-9f);
                 nrx = gx_ * inv;
                 nry = gy_ * inv;
                 nrz = gz_ * inv;
@@ -258,52 +385,8 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         const float stCurv = -p.sigma * curvature * ind;
         const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
 
-        // ---- moments (src/lbm.cu:33-128)
         const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
         const Nbr nb = make_nbr<!JET>(x, y, z, nx, ny, nz);
-        float pop[Q];
-        static_for<Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            if constexpr (PAR == 0) {
-                pop[q] = Pop<HALF>::ld(p.f, p.N, q, c);
-            } else {
-                pop[q] = Pop<HALF>::ld(p.f, p.N, opp(q), nb_idx<q, -1>(nb));
-            }
-        });
-        float rho = 0.0f;
-#pragma unroll
-        for (int q = 0; q < Q; ++q) rho += pop[q];
-        rho += 1.0f;
-        const float invRho = 1.0f / rho;
-        auto pv = [&](auto qc) { return pop[decltype(qc)::value]; };
-        float ux = invRho * pair_chain<Q, 0, -1>(pv);
-        float uy = invRho * pair_chain<Q, 1, -1>(pv);
-        float uz = invRho * pair_chain<Q, 2, -1>(pv);
-        ux += fsx * 0.5f * invRho;
-        uy += fsy * 0.5f * invRho;
-        uz += fsz * 0.5f * invRho;
-        const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
-        float pxx = 0.0f, pyy = 0.0f, pzz = 0.0f, pxy = 0.0f, pxz = 0.0f, pyz = 0.0f;
-        static_for<Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const float cu = cu_of<Q, q>(ux, uy, uz);
-            const float feq = f_eq<Q, q>(rho, uu, cu);
-            const float frc = force_of<Q, q>(cu, ux, uy, uz, fsx, fsy, fsz);
-            const float fneq = pop[q] - feq + 0.5f * frc;
-            if constexpr (kCX[q] != 0) pxx += fneq;
-            if constexpr (kCY[q] != 0) pyy += fneq;
-            if constexpr (kCZ[q] != 0) pzz += fneq;
-            if constexpr (kCX[q] * kCY[q] > 0) pxy += fneq;
-            if constexpr (kCX[q] * kCY[q] < 0) pxy -= fneq;
-            if constexpr (kCX[q] * kCZ[q] > 0) pxz += fneq;
-            if constexpr (kCX[q] * kCZ[q] < 0) pxz -= fneq;
-            if constexpr (kCY[q] * kCZ[q] > 0) pyz += fneq;
-            if constexpr (kCY[q] * kCZ[q] < 0) pyz -= fneq;
-        });
-        const float trace = pxx + pyy + pzz;
-        pxx -= kCs2 * trace;
-        pyy -= kCs2 * trace;
-        pzz -= kCs2 * trace;
 
         // ---- relaxation (src/lbm.cu:158-172)
         float omega;
@@ -316,42 +399,207 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             const float visc_local = (1.0f - phi) * p.visc_water + phi * p.visc_oil;
             const float tau_zmax = 0.5f + kAs2 * (visc_local * (1.0f + 100.0f));
             const float tau_eff = tau_phi + r * (tau_zmax - tau_phi);
-            omega = 1.0f / tau_eff;
+            omega = recip(tau_eff);
         } else {
             omega = p.omega_ref;
         }
         const float omco = 1.0f - omega;
 
-        // ---- collide and stream f (src/lbm.cu:176-207)
-        static_for<Q>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            const float cu = cu_of<Q, q>(ux, uy, uz);
-            const float feq = f_eq<Q, q>(rho, uu, cu);
-            const float frc = force_of<Q, q>(cu, ux, uy, uz, fsx, fsy, fsz);
-            const float fneq = f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, ux, uy, uz);
-            const float fo = feq + omco * fneq + 0.5f * frc;
-            if constexpr (PAR == 0) {
-                Pop<HALF>::st(p.f, p.N, opp(q), c, fo);
-            } else {
-                Pop<HALF>::st(p.f, p.N, q, nb_idx<q, 1>(nb), fo);
+        float rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz;
+        if constexpr (LIT) {
+            // ---- literal evaluation, direction by direction (src/lbm.cu:33-207)
+            float pop[Q];
+            static_for<Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (PAR == 0) {
+                    pop[q] = Pop<HALF>::ld(p.f, p.sN, q, c);
+                } else {
+                    pop[q] = Pop<HALF>::ld(p.f, p.sN, opp(q), nb_idx<q, -1>(nb));
+                }
+            });
+            rho = 0.0f;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) rho += pop[q];
+            rho += 1.0f;
+            const float invRho = 1.0f / rho;
+            auto pv = [&](auto qc) { return pop[decltype(qc)::value]; };
+            ux = invRho * pair_chain<Q, 0, -1>(pv);
+            uy = invRho * pair_chain<Q, 1, -1>(pv);
+            uz = invRho * pair_chain<Q, 2, -1>(pv);
+            ux += fsx * 0.5f * invRho;
+            uy += fsy * 0.5f * invRho;
+            uz += fsz * 0.5f * invRho;
+            const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+            pxx = pyy = pzz = pxy = pxz = pyz = 0.0f;
+            static_for<Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const float cu = cu_of<Q, q>(ux, uy, uz);
+                const float feq = f_eq<Q, q>(rho, uu, cu);
+                const float frc = force_of<Q, q>(cu, ux, uy, uz, fsx, fsy, fsz);
+                const float fneq = pop[q] - feq + 0.5f * frc;
+                if constexpr (kCX[q] != 0) pxx += fneq;
+                if constexpr (kCY[q] != 0) pyy += fneq;
+                if constexpr (kCZ[q] != 0) pzz += fneq;
+                if constexpr (kCX[q] * kCY[q] > 0) pxy += fneq;
+                if constexpr (kCX[q] * kCY[q] < 0) pxy -= fneq;
+                if constexpr (kCX[q] * kCZ[q] > 0) pxz += fneq;
+                if constexpr (kCX[q] * kCZ[q] < 0) pxz -= fneq;
+                if constexpr (kCY[q] * kCZ[q] > 0) pyz += fneq;
+                if constexpr (kCY[q] * kCZ[q] < 0) pyz -= fneq;
+            });
+            const float trace = pxx + pyy + pzz;
+            pxx -= kCs2 * trace;
+            pyy -= kCs2 * trace;
+            pzz -= kCs2 * trace;
+            static_for<Q>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                const float cu = cu_of<Q, q>(ux, uy, uz);
+                const float feq = f_eq<Q, q>(rho, uu, cu);
+                const float frc = force_of<Q, q>(cu, ux, uy, uz, fsx, fsy, fsz);
+                const float fneq = f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, ux, uy, uz);
+                const float fo = feq + omco * fneq + 0.5f * frc;
+                if constexpr (PAR == 0) {
+                    Pop<HALF>::st(p.f, p.sN, opp(q), c, fo);
+                } else {
+                    Pop<HALF>::st(p.f, p.sN, q, nb_idx<q, 1>(nb), fo);
+                }
+            });
+        } else {
+            // ---- pair-symmetric evaluation
+            // Loads: pair j holds directions a = 2j+1 (c) and b = 2j+2 (-c).
+            float fa[NP], fb[NP];
+            const float f0 = Pop<HALF>::ld(p.f, p.sN, 0, c);
+            static_for<NP>([&](auto jc) {
+                constexpr int j = decltype(jc)::value, a = 2 * j + 1, b = 2 * j + 2;
+                if constexpr (PAR == 0) {
+                    fa[j] = Pop<HALF>::ld(p.f, p.sN, a, c);
+                    fb[j] = Pop<HALF>::ld(p.f, p.sN, b, c);
+                } else {
+                    fa[j] = Pop<HALF>::ld(p.f, p.sN, b, nb_idx<a, -1>(nb));
+                    fb[j] = Pop<HALF>::ld(p.f, p.sN, a, nb_idx<a, 1>(nb));
+                }
+            });
+            // Moments from pair sums and differences: rho = 1 + sum f,
+            // j = sum f c, Pi = sum f c c (src/lbm.cu:49-115).
+            float sp[NP], dp[NP];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                sp[j] = fa[j] + fb[j];
+                dp[j] = fa[j] - fb[j];
             }
-        });
+            float m0 = f0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) m0 += sp[j];
+            rho = m0 + 1.0f;
+            float jx = 0.0f, jy = 0.0f, jz = 0.0f, Pi[6] = {0, 0, 0, 0, 0, 0};
+            static_for<NP>([&](auto jc) {
+                constexpr int j = decltype(jc)::value, a = 2 * j + 1;
+                constexpr int cx = kCX[a], cy = kCY[a], cz = kCZ[a];
+                if constexpr (cx != 0) { jx = cx > 0 ? jx + dp[j] : jx - dp[j]; Pi[0] += sp[j]; }
+                if constexpr (cy != 0) { jy = cy > 0 ? jy + dp[j] : jy - dp[j]; Pi[1] += sp[j]; }
+                if constexpr (cz != 0) { jz = cz > 0 ? jz + dp[j] : jz - dp[j]; Pi[2] += sp[j]; }
+                if constexpr (cx * cy != 0) Pi[3] = cx * cy > 0 ? Pi[3] + sp[j] : Pi[3] - sp[j];
+                if constexpr (cx * cz != 0) Pi[4] = cx * cz > 0 ? Pi[4] + sp[j] : Pi[4] - sp[j];
+                if constexpr (cy * cz != 0) Pi[5] = cy * cz > 0 ? Pi[5] + sp[j] : Pi[5] - sp[j];
+            });
+            const float invRho = recip(rho);
+            ux = (jx + 0.5f * fsx) * invRho;
+            uy = (jy + 0.5f * fsy) * invRho;
+            uz = (jz + 0.5f * fsz) * invRho;
+            // Deviatoric non-equilibrium stress: Pi minus the equilibrium
+            // second moment rho u u (+ isotropic part) plus half the Guo
+            // force moment F u + u F.
+            const float axx = Pi[0] - rho * ux * ux + fsx * ux;
+            const float ayy = Pi[1] - rho * uy * uy + fsy * uy;
+            const float azz = Pi[2] - rho * uz * uz + fsz * uz;
+            pxy = Pi[3] - rho * ux * uy + 0.5f * (fsx * uy + fsy * ux);
+            pxz = Pi[4] - rho * ux * uz + 0.5f * (fsx * uz + fsz * ux);
+            pyz = Pi[5] - rho * uy * uz + 0.5f * (fsy * uz + fsz * uy);
+            const float tr3 = (axx + ayy + azz) * kCs2;
+            pxx = axx - tr3;
+            pyy = ayy - tr3;
+            pzz = azz - tr3;
 
-        // ---- phase populations (src/lbm.cu:209-238)
-        const float sharp = p.gamma * phi * (1.0f - phi);
-        static_for<7>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            float go = g_post<q>(phi, ux, uy, uz, sharp, nrx, nry, nrz);
-            if constexpr (HALF) go -= p.wphi0[q];
-            if constexpr (PAR == 0) {
-                Pop<HALF>::st(p.g, p.N, opp(q), c, go);
-            } else {
-                Pop<HALF>::st(p.g, p.N, q, nb_idx<q, 1>(nb), go);
+            const float usq = ux * ux + uy * uy + uz * uz;
+            const float uF = ux * fsx + uy * fsy + uz * fsz;
+            Coll k;
+            k.V = 1.0f - 1.5f * usq;
+            k.B0 = rho * k.V - 1.0f - 1.5f * uF;
+            k.R3 = 3.0f * rho;
+            k.R45 = 4.5f * rho;
+            k.O45 = 4.5f * omco;
+            k.Pxx = pxx; k.Pyy = pyy; k.Pzz = pzz;
+            k.Sxy = pxx + pyy; k.Sxz = pxx + pzz; k.Syz = pyy + pzz;
+            k.S3 = k.Sxy + pzz;
+            k.Txy = 2.0f * pxy; k.Txz = 2.0f * pxz; k.Tyz = 2.0f * pyz;
+            if constexpr (Q == 27) {
+                k.X1 = pxx * uy + 2.0f * ux * pxy;
+                k.X2 = pxx * uz + 2.0f * ux * pxz;
+                k.X3 = pyy * ux + 2.0f * uy * pxy;
+                k.X4 = pyy * uz + 2.0f * uy * pyz;
+                k.X5 = pzz * ux + 2.0f * uz * pxz;
+                k.X6 = pzz * uy + 2.0f * uz * pyz;
+                k.Bx = k.X3 + k.X5;
+                k.By = k.X1 + k.X6;
+                k.Bz = k.X2 + k.X4;
+                k.X7s = 6.0f * (ux * pyz + uy * pxz + uz * pxy);
             }
-        });
+            // rest direction: f_eq + F_0/2 (its regularized part is -cs^2 tr P = 0)
+            Pop<HALF>::st(p.f, p.sN, 0, c, wk<Q>(0) * k.B0);
+            static_for<NP>([&](auto jc) {
+                constexpr int j = decltype(jc)::value, a = 2 * j + 1, b = 2 * j + 2;
+                float op, om;
+                pair_out<Q, kCX[a], kCY[a], kCZ[a]>(k, ux, uy, uz, fsx, fsy, fsz, op, om);
+                if constexpr (PAR == 0) {
+                    Pop<HALF>::st(p.f, p.sN, b, c, op);
+                    Pop<HALF>::st(p.f, p.sN, a, c, om);
+                } else {
+                    Pop<HALF>::st(p.f, p.sN, a, nb_idx<a, 1>(nb), op);
+                    Pop<HALF>::st(p.f, p.sN, b, nb_idx<a, -1>(nb), om);
+                }
+            });
+        }
+
+        // ---- phase populations (src/lbm.cu:209-238): g_eq + sharpening flux
+        const float sharp = p.gamma * phi * (1.0f - phi);
+        if constexpr (LIT) {
+            static_for<7>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                float go = g_post<q>(phi, ux, uy, uz, sharp, nrx, nry, nrz);
+                if constexpr (HALF) go -= p.wphi0[q];
+                if constexpr (PAR == 0) {
+                    Pop<HALF>::st(p.g, p.sN, opp(q), c, go);
+                } else {
+                    Pop<HALF>::st(p.g, p.sN, q, nb_idx<q, 1>(nb), go);
+                }
+            });
+        } else {
+            const float ge = 0.125f * phi;
+            const float p4 = 4.0f * phi;
+            const float go_[3] = {0.125f * (p4 * ux + sharp * nrx), 0.125f * (p4 * uy + sharp * nry),
+                                  0.125f * (p4 * uz + sharp * nrz)};
+            float g0 = 0.25f * phi;
+            if constexpr (HALF) g0 -= p.wphi0[0];
+            Pop<HALF>::st(p.g, p.sN, 0, c, g0);
+            static_for<3>([&](auto ac) {
+                constexpr int ax = decltype(ac)::value, a = 2 * ax + 1, b = 2 * ax + 2;
+                float gp = ge + go_[ax], gm = ge - go_[ax];
+                if constexpr (HALF) {
+                    gp -= p.wphi0[a];
+                    gm -= p.wphi0[b];
+                }
+                if constexpr (PAR == 0) {
+                    Pop<HALF>::st(p.g, p.sN, b, c, gp);
+                    Pop<HALF>::st(p.g, p.sN, a, c, gm);
+                } else {
+                    Pop<HALF>::st(p.g, p.sN, a, nb_idx<a, 1>(nb), gp);
+                    Pop<HALF>::st(p.g, p.sN, b, nb_idx<a, -1>(nb), gm);
+                }
+            });
+        }
 
         // ---- macroscopic fields for output and for the jet boundary planes
-        if (p.save) {
+        if constexpr (SAVE) {
             float* const* F = p.fld;
             F[0][c] = rho; F[1][c] = ux; F[2][c] = uy; F[3][c] = uz;
             F[4][c] = pxx; F[5][c] = pyy; F[6][c] = pzz; F[7][c] = pxy; F[8][c] = pxz; F[9][c] = pyz;
@@ -364,7 +612,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 float* b = p.bcp + (z == 1 ? 0u : (uint32_t)kPlaneVals * plane) + (uint32_t)x + (uint32_t)nx * (uint32_t)y;
                 const float v[kPlaneVals] = {rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, fsx, fsy, fsz};
 #pragma unroll
-                for (int k = 0; k < kPlaneVals; ++k) b[(uint32_t)k * plane] = v[k];
+                for (int kk = 0; kk < kPlaneVals; ++kk) b[(uint32_t)kk * plane] = v[kk];
             }
         }
     }
@@ -429,7 +677,7 @@ template <int Q, bool HALF, int NPAR>
 __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
 #pragma clang fp contract(off)
     const int x = 1 + blockIdx.x * kTX + (threadIdx.x & (kTX - 1));
-    const int y = 1 + blockIdx.y * kTY + (threadIdx.x >> 6);
+    const int y = 1 + blockIdx.y * kTY + (int)(threadIdx.x / kTX);
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     if (x > nx - 2 || y > ny - 2) return;
     const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
@@ -459,7 +707,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                 int slot;
                 uint32_t i;
                 in_slot<q, NPAR>(nb, c, slot, i);
-                Pop<HALF>::st(p.f, p.N, slot, i, val);
+                Pop<HALF>::st(p.f, p.sN, slot, i, val);
             }
         });
         float g5 = 0.0f;
@@ -478,7 +726,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         int slot;
         uint32_t i;
         in_slot<5, NPAR>(nb, c, slot, i);
-        Pop<HALF>::st(p.g, p.N, slot, i, HALF ? g5 - p.wphi0[5] : g5);
+        Pop<HALF>::st(p.g, p.sN, slot, i, HALF ? g5 - p.wphi0[5] : g5);
     }
 
     // ---------------- outflow, target cell (x, y, nz-2) ----------------
@@ -515,7 +763,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                 int slot;
                 uint32_t i;
                 in_slot<q, NPAR>(nb, c, slot, i);
-                Pop<HALF>::st(p.f, p.N, slot, i, val);
+                Pop<HALF>::st(p.f, p.sN, slot, i, val);
             }
         });
         const float phi_b = phi_g[c];
@@ -524,7 +772,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         int slot;
         uint32_t i;
         in_slot<6, NPAR>(nb, c, slot, i);
-        Pop<HALF>::st(p.g, p.N, slot, i, HALF ? g6 - p.wphi0[6] : g6);
+        Pop<HALF>::st(p.g, p.sN, slot, i, HALF ? g6 - p.wphi0[6] : g6);
         // zero-gradient ghost plane z = nz-1
         const uint32_t cg = c + plane;
         F[10][cg] = phi_b;
@@ -538,52 +786,47 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// launchers
+// launchers (instantiated per velocity set in step_q19.hip / step_q27.hip)
 // ---------------------------------------------------------------------------
 template <bool JET, bool HALF, int PAR>
 static hipError_t phase_sum_t(const KParams& p, hipStream_t s) {
-    dim3 grid((p.nx + kTX - 1) / kTX, (p.ny + kTY - 1) / kTY, p.nz - 2);
+    dim3 grid((p.nx + kTX - 1) / kTX, (p.ny + kTY - 1) / kTY, (p.nz - 2 + kPsZ - 1) / kPsZ);
     hipLaunchKernelGGL((k_phase_sum<JET, HALF, PAR>), grid, dim3(kThreads), 0, s, p);
     return hipGetLastError();
 }
 
-hipError_t launch_phase_sum(const KParams& p, int Q, bool jet, bool half, int parity, hipStream_t s) {
-    (void)Q;
-    if (jet) {
-        if (half) return parity ? phase_sum_t<true, true, 1>(p, s) : phase_sum_t<true, true, 0>(p, s);
-        return parity ? phase_sum_t<true, false, 1>(p, s) : phase_sum_t<true, false, 0>(p, s);
-    }
-    if (half) return parity ? phase_sum_t<false, true, 1>(p, s) : phase_sum_t<false, true, 0>(p, s);
-    return parity ? phase_sum_t<false, false, 1>(p, s) : phase_sum_t<false, false, 0>(p, s);
-}
-
-template <int Q, bool JET, bool HALF, bool MLP>
+template <int Q, bool JET, bool HALF, bool MLP, bool LIT, bool SAVE>
 static hipError_t step_t(const KParams& p, int parity, hipStream_t s) {
     const uint32_t nblk = (uint32_t)p.ntx * (uint32_t)p.nty * (uint32_t)p.ntz;
     if (parity)
-        hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP>), dim3(nblk), dim3(kThreads), 0, s, p);
+        hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE>), dim3(nblk), dim3(kThreads), 0, s, p);
     else
-        hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP>), dim3(nblk), dim3(kThreads), 0, s, p);
+        hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE>), dim3(nblk), dim3(kThreads), 0, s, p);
     return hipGetLastError();
 }
 
 template <int Q, bool JET, bool HALF>
-static hipError_t step_m(const KParams& p, int parity, bool mlp, hipStream_t s) {
-    if (mlp) {
-        if constexpr (kMlpKernels) return step_t<Q, JET, HALF, true>(p, parity, s);
+static hipError_t step_v(const KParams& p, int parity, int variant, bool save, hipStream_t s) {
+    if (variant == 1) {  // literal evaluation (FP32 and FP16 storage)
+        return save ? step_t<Q, JET, HALF, false, true, true>(p, parity, s)
+                    : step_t<Q, JET, HALF, false, true, false>(p, parity, s);
+    }
+    if (variant == 2) {
+        if constexpr (kMlpKernels) {
+            return save ? step_t<Q, JET, HALF, true, false, true>(p, parity, s)
+                        : step_t<Q, JET, HALF, true, false, false>(p, parity, s);
+        }
         return hipErrorNotSupported;
     }
-    return step_t<Q, JET, HALF, false>(p, parity, s);
+    return save ? step_t<Q, JET, HALF, false, false, true>(p, parity, s)
+                : step_t<Q, JET, HALF, false, false, false>(p, parity, s);
 }
 
 template <int Q>
-static hipError_t step_q(const KParams& p, bool jet, bool half, int parity, bool mlp, hipStream_t s) {
-    if (jet) return half ? step_m<Q, true, true>(p, parity, mlp, s) : step_m<Q, true, false>(p, parity, mlp, s);
-    return half ? step_m<Q, false, true>(p, parity, mlp, s) : step_m<Q, false, false>(p, parity, mlp, s);
-}
-
-hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, bool mlp, hipStream_t s) {
-    return Q == 27 ? step_q<27>(p, jet, half, parity, mlp, s) : step_q<19>(p, jet, half, parity, mlp, s);
+static hipError_t step_q(const KParams& p, bool jet, bool half, int parity, int variant, hipStream_t s) {
+    const bool save = p.save != 0;
+    if (jet) return half ? step_v<Q, true, true>(p, parity, variant, save, s) : step_v<Q, true, false>(p, parity, variant, save, s);
+    return half ? step_v<Q, false, true>(p, parity, variant, save, s) : step_v<Q, false, false>(p, parity, variant, save, s);
 }
 
 template <int Q, bool HALF>
@@ -594,11 +837,6 @@ static hipError_t jet_bc_t(const KParams& p, int npar, hipStream_t s) {
     else
         hipLaunchKernelGGL((k_jet_bc<Q, HALF, 0>), grid, dim3(kThreads), 0, s, p);
     return hipGetLastError();
-}
-
-hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hipStream_t s) {
-    if (Q == 27) return half ? jet_bc_t<27, true>(p, next_parity, s) : jet_bc_t<27, false>(p, next_parity, s);
-    return half ? jet_bc_t<19, true>(p, next_parity, s) : jet_bc_t<19, false>(p, next_parity, s);
 }
 
 }  // namespace pf

@@ -4,7 +4,8 @@ Workload (one "step" = one full timestep of the whole grid): round jet,
 D3Q27 regularized hydrodynamics + D3Q7 phase field, 256x256x1024 cells,
 nozzle D=32, u=0.05, Re=5000, We=500, FP32 populations.  Inputs are
 resident in HBM before timing starts.  Also reported: the FP16-storage form
-of the same workload, the Laplace pressure error of static droplets
+of the same workload, the same workload with the learned curvature closure
+(its cost in MLUPS), the Laplace pressure error of static droplets
 (R = 8, 12, 16) and the CPU oracle timed on the host on a bounded sample.
 
 MLUPS counts nx*ny*nz cells per step, as the reference does
@@ -36,8 +37,10 @@ def kernel_bytes_per_cell(q, s):
     return 2 * q * s + 7 * s + 4
 
 
-def run_form(pf, cfg, steps, warmup):
+def run_form(pf, cfg, steps, warmup, closure_params=None):
     sim = pf.Simulation(cfg)
+    if closure_params is not None:
+        sim.set_closure_params(closure_params)
     sim.init()
     sim.step(warmup)
     sim.sync()
@@ -167,6 +170,17 @@ def main():
                                "step_roofline_frac": round(mlups16 * 1e6 * b16 / (HBM_PEAK_GBS * 1e9), 4),
                                "kernel_ms": round(tm16["step_ms"] / args.steps, 4), "bytes_per_update": b16}
         log(f"[bench] fp16 storage: {mlups16:.0f} MLUPS ({mlups16 / mlups:.2f}x)")
+        from phasefieldlbm_amd import closure
+        cm = presets.jet_d3q27_256x256x1024()
+        cm.closure = pf.CLOSURE_MLP
+        wallm, tmm, diagm = run_form(pf, cm, args.steps, args.warmup, closure_params=closure.load())
+        mlupsm = cells * args.steps / wallm / 1e6
+        out["learned_closure"] = {"value": round(mlupsm, 1), "unit": "MLUPS",
+                                  "cost_frac": round(1.0 - mlupsm / mlups, 4),
+                                  "phase_and_closure_ms": round(tmm["phase_sum_ms"] / args.steps, 4),
+                                  "step_ms": round(tmm["step_ms"] / args.steps, 4),
+                                  "umax": diagm["umax"]}
+        log(f"[bench] learned closure: {mlupsm:.0f} MLUPS (cost {100 * (1 - mlupsm / mlups):.1f} %)")
         out["laplace"] = [laplace(pf, presets, R) for R in (8, 12, 16)]
         log(f"[bench] laplace: {[round(r['rel_err'], 4) for r in out['laplace']]}")
 

@@ -114,6 +114,7 @@ typedef struct pf_config {
     float mlp_lo, mlp_hi;  /* MLP applies where mlp_lo < phi < mlp_hi */
     int32_t kernel;        /* pf_kernel */
     int32_t slot_pad;      /* cells of padding between population slots; <0 = automatic */
+    float mlp_kmin, mlp_kmax; /* MLP applies where the FD curvature lies in [kmin, kmax] */
 } pf_config;
 
 typedef struct pf_ctx pf_ctx;
@@ -169,9 +170,41 @@ int pf_phase_mass(pf_ctx *ctx, double *mass);
 
 /* Learned-closure weights, fp32 host arrays in row-major [out][in]:
  * W1[64*27], b1[64], W2[64*64], b2[64], W3[64], b3[1].  Stored on the
- * device as BF16 weights and FP32 biases. */
+ * device as BF16 weights and FP32 biases.  The network maps the 3x3x3 phi
+ * stencil (input 2*phi - 1, feature index (dz+1)*9 + (dy+1)*3 + (dx+1)) to
+ * 4 x curvature.  No counterpart in the reference. */
 int pf_set_closure(pf_ctx *ctx, const float *W1, const float *b1, const float *W2,
                    const float *b2, const float *W3, const float *b3);
+
+/*
+ * Training of the learned closure on the matrix cores (no counterpart in the
+ * reference).  Parameters are one FP32 vector of PF_MLP_PARAMS values:
+ * W1[64][27] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.  Samples are 27
+ * stencil features (2*phi - 1) and a target (4 x curvature).
+ */
+#define PF_MLP_PARAMS 6017
+typedef struct pf_mlp pf_mlp;
+int pf_mlp_create(int32_t device, pf_mlp **out);
+void pf_mlp_destroy(pf_mlp *m);
+int pf_mlp_set_params(pf_mlp *m, const float *params);
+int pf_mlp_get_params(pf_mlp *m, float *params);
+/* Copy n samples (x: n*27 floats, stored as BF16; t: n floats) to the device. */
+int pf_mlp_set_data(pf_mlp *m, const float *x, const float *t, size_t n);
+/* Adam (beta 0.9/0.999) on the mean squared error over minibatches of
+ * `batch` samples (a multiple of 64), shuffled each epoch from `seed`; with
+ * augment != 0 every sample is seen under a random one of the 48 symmetries
+ * of the cube.  epoch_loss[e] = mean squared error seen during epoch e. */
+int pf_mlp_train(pf_mlp *m, uint32_t epochs, uint32_t batch, float lr, uint32_t seed, int32_t augment,
+                 float *epoch_loss);
+/* Gradient of the mean squared error over the first n samples (one forward
+ * and backward pass, no update): grad[PF_MLP_PARAMS], *mse. */
+int pf_mlp_gradients(pf_mlp *m, size_t n, float *grad, double *mse);
+/* Network output for n samples x[n*27]. */
+int pf_mlp_predict(pf_mlp *m, const float *x, size_t n, float *y);
+/* One 16x16x32 BF16 MFMA tile, C = A B with A 16x32 and B 32x16 row-major
+ * (rounded to BF16), as the kernels lay the operands out; for verifying the
+ * lane layout. */
+int pf_mfma_check(int32_t device, const float *a, const float *b, float *c);
 
 /* Time `nsteps` steps (flags as pf_step) with HIP events on the context's
  * stream.  out_ms[0] = total, out_ms[1] = phase-sum kernel total,

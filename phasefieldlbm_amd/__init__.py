@@ -22,7 +22,8 @@ from ._abi import (CLOSURE_FD, CLOSURE_MLP, D3Q19, D3Q27, DROPLET, FIELDS, JET, 
 
 __all__ = ["Config", "Simulation", "PfError", "presets", "FIELDS", "D3Q19", "D3Q27", "JET", "DROPLET",
            "STORE_FP32", "STORE_FP16", "NOISE_GAUSSIAN_XY", "NOISE_UNIFORM_Z", "NOISE_NONE",
-           "CLOSURE_FD", "CLOSURE_MLP", "KERNEL_PAIR", "KERNEL_LITERAL", "library_path", "copy_bandwidth"]
+           "CLOSURE_FD", "CLOSURE_MLP", "KERNEL_PAIR", "KERNEL_LITERAL", "library_path", "copy_bandwidth", "MLP",
+           "split_params", "mfma_check"]
 
 
 def library_path() -> str:
@@ -77,6 +78,8 @@ class Config:
     mlp_hi: float = 0.95
     kernel: int = KERNEL_PAIR
     slot_pad: int = -1
+    mlp_kmin: float = -0.6
+    mlp_kmax: float = -0.06
 
     @classmethod
     def reference(cls, flow_case: int, velocity_set: int) -> "Config":
@@ -194,6 +197,10 @@ class Simulation:
         self._check(self._lib.pf_phase_mass(self._h, ctypes.byref(m)), "pf_phase_mass")
         return m.value
 
+    def set_closure_params(self, params):
+        """Closure weights from one 6017-value parameter vector."""
+        self.set_closure(*split_params(params))
+
     def set_closure(self, W1, b1, W2, b2, W3, b3):
         arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in (W1, b1, W2, b2, W3, b3)]
         sizes = [64 * 27, 64, 64 * 64, 64, 64, 1]
@@ -212,6 +219,96 @@ class Simulation:
 
     def device_bytes(self) -> int:
         return int(self._lib.pf_device_bytes(self._h))
+
+
+class MLP:
+    """The learned curvature closure's network on the device (pf_mlp).
+
+    Parameters are one FP32 vector of 6017 values, W1[64][27] | b1 | W2[64][64]
+    | b2 | w3[64] | b3; samples are 27 stencil features (2*phi - 1, feature
+    (dz+1)*9 + (dy+1)*3 + (dx+1)) and a target 4*curvature.
+    """
+
+    NPARAMS = _abi.MLP_PARAMS
+
+    def __init__(self, device: int = 0, params: Optional[np.ndarray] = None):
+        self._lib = _abi.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.pf_mlp_create(device, ctypes.byref(h))
+        if rc:
+            raise PfError(rc, "pf_mlp_create")
+        self._h = h
+        if params is not None:
+            self.set_params(params)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pf_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            raise PfError(rc, what)
+
+    def set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        assert a.size == self.NPARAMS
+        self._check(self._lib.pf_mlp_set_params(self._h, a.ctypes.data), "pf_mlp_set_params")
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.NPARAMS, dtype=np.float32)
+        self._check(self._lib.pf_mlp_get_params(self._h, out.ctypes.data), "pf_mlp_get_params")
+        return out
+
+    def set_data(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 27)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        assert x.shape[0] == t.size
+        self._keep = (x, t)
+        self._check(self._lib.pf_mlp_set_data(self._h, x.ctypes.data, t.ctypes.data, t.size), "pf_mlp_set_data")
+
+    def train(self, epochs: int, batch: int = 512, lr: float = 1e-3, seed: int = 1, augment: bool = True):
+        out = np.zeros(epochs, dtype=np.float32)
+        self._check(self._lib.pf_mlp_train(self._h, epochs, batch, lr, seed, 1 if augment else 0,
+                                           out.ctypes.data), "pf_mlp_train")
+        return out
+
+    def gradients(self, n: int):
+        g = np.empty(self.NPARAMS, dtype=np.float32)
+        mse = ctypes.c_double()
+        self._check(self._lib.pf_mlp_gradients(self._h, n, g.ctypes.data, ctypes.byref(mse)), "pf_mlp_gradients")
+        return g, mse.value
+
+    def predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 27)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        self._check(self._lib.pf_mlp_predict(self._h, x.ctypes.data, x.shape[0], y.ctypes.data), "pf_mlp_predict")
+        return y
+
+
+def split_params(params):
+    """(W1[64,27], b1, W2[64,64], b2, w3[64], b3[1]) views of a parameter vector."""
+    p = np.asarray(params, dtype=np.float32).reshape(-1)
+    o = [0, 64 * 27, 64 * 27 + 64, 64 * 27 + 64 + 4096, 64 * 27 + 128 + 4096, 64 * 27 + 192 + 4096, 6017]
+    return (p[o[0]:o[1]].reshape(64, 27), p[o[1]:o[2]], p[o[2]:o[3]].reshape(64, 64), p[o[3]:o[4]],
+            p[o[4]:o[5]], p[o[5]:o[6]])
+
+
+def mfma_check(a, b, device: int = 0) -> np.ndarray:
+    """One BF16 MFMA tile C = A B (A 16x32, B 32x16) computed as the kernels do."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    c = np.empty((16, 16), dtype=np.float32)
+    rc = _abi.load().pf_mfma_check(device, a.ctypes.data, b.ctypes.data, c.ctypes.data)
+    if rc:
+        raise PfError(rc, "pf_mfma_check")
+    return c
 
 
 from . import presets  # noqa: E402

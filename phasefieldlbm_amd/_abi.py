@@ -20,6 +20,7 @@ NOISE_GAUSSIAN_XY, NOISE_UNIFORM_Z, NOISE_NONE = 0, 1, 2
 CLOSURE_FD, CLOSURE_MLP = 0, 1
 KERNEL_PAIR, KERNEL_LITERAL = 0, 1
 STEP_SAVE_LAST = 1
+MLP_PARAMS = 6017
 
 # host::FieldID order (src/fileIO/fields.cuh:33-68)
 FIELDS = ("rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz",
@@ -59,6 +60,7 @@ class PfConfig(ctypes.Structure):
         ("mlp_lo", ctypes.c_float), ("mlp_hi", ctypes.c_float),
         ("kernel", ctypes.c_int32),
         ("slot_pad", ctypes.c_int32),
+        ("mlp_kmin", ctypes.c_float), ("mlp_kmax", ctypes.c_float),
     ]
 
 
@@ -81,6 +83,17 @@ SIGNATURES = {
     "pf_device_bytes": (ctypes.c_size_t, [ctypes.c_void_p]),
     "pf_copy_bandwidth": (ctypes.c_int, [ctypes.c_int32, ctypes.c_size_t, ctypes.c_uint32,
                                          ctypes.POINTER(ctypes.c_double)]),
+    "pf_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "pf_mlp_destroy": (None, [ctypes.c_void_p]),
+    "pf_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "pf_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "pf_mlp_set_data": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "pf_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                    ctypes.c_uint32, ctypes.c_int32, ctypes.c_void_p]),
+    "pf_mlp_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                        ctypes.POINTER(ctypes.c_double)]),
+    "pf_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "pf_mfma_check": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "pf_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -71,9 +71,11 @@ struct KParams {
     uint32_t seed;
     uint32_t t;  // time index of this step (inflow)
     // learned closure
-    const uint16_t* mlp_w;  // bf16: W1[64x27] | W2[64x64] | W3[64]
+    const uint16_t* mlp_w;  // bf16: W1[64x32] (27 used) | W2[64x64] | w3[64]
     const float* mlp_b;     // fp32: b1[64] | b2[64] | b3[1]
-    float mlp_lo, mlp_hi;
+    float mlp_lo, mlp_hi;   // phi window of the closure
+    float mlp_kmin, mlp_kmax;  // FD-curvature window of the closure
+    float* kappa;              // closure curvature at interface cells [N]
 };
 
 // Launchers; all return hipError_t of the launch.
@@ -92,5 +94,15 @@ hipError_t launch_phase_mass(const KParams& p, bool jet, bool half, int parity, 
                              hipStream_t s);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
+hipError_t launch_mlp_grad(const uint16_t* w16, const float* bias, const float* xs, const float* ts, const int* idx,
+                           const uint8_t* symperm, int nsym, uint32_t sym_seed, size_t n, size_t s_begin,
+                           size_t count, float scale, float* grad_out, hipStream_t s);
+hipError_t launch_mlp_adam(const float* parts, int nparts, float* master, float* m1, float* m2, uint16_t* w16,
+                           float* bias, float lr, int step, int apply, float* grad_sum, double* loss_acc,
+                           hipStream_t s);
+hipError_t launch_mlp_forward(const uint16_t* w16, const float* bias, const float* xs, size_t n, float* y,
+                              hipStream_t s);
+hipError_t launch_mfma_layout(const uint16_t* a, const uint16_t* b, float* d, hipStream_t s);
+hipError_t launch_closure(const KParams& p, hipStream_t s);
 
 }  // namespace pf

@@ -1,4 +1,17 @@
-// Learned curvature closure inside the fused step kernel.
+// Learned curvature closure: a 27 -> 64 -> 64 -> 1 MLP with GELU, BF16
+// weights and FP32 accumulation on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16).  No counterpart in the reference, whose
+// curvature is always the finite-difference div(n) (src/phaseField.cuh:142-219).
+//
+// MFMA 16x16x32 lane layout (lane = i + 16 g, i in 0..15, g in 0..3):
+//   A (16 x 32): lane holds A[i][8g .. 8g+7]
+//   B (32 x 16): lane holds B[8g .. 8g+7][i]
+//   D (16 x 16): lane holds D[4g + r][i], r = 0..3
+// Activations are kept transposed (hidden unit x sample), so the D tiles of
+// one layer are the B operand of the next without leaving registers: for
+// the K-block s of the second layer, lane group g supplies hidden units
+// {32s + 4g + r, 32s + 16 + 4g + r}, and the W2 fragment is gathered in that
+// same order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,18 +20,102 @@
 
 namespace pf {
 
-constexpr bool kMlpKernels = false;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <bool MLP>
-struct MlpShared {};
+constexpr int kMlpInPad = 32;                                   // 27 features + zero padding
+constexpr int kMlpW1 = kMlpHidden * kMlpInPad;                  // bf16 W1 [64][32]
+constexpr int kMlpW2 = kMlpHidden * kMlpHidden;                 // bf16 W2 [64][64]
+constexpr int kMlpWords = kMlpW1 + kMlpW2 + kMlpHidden;         // + w3 [64]
+constexpr int kMlpBias = 2 * kMlpHidden + 1;                    // b1, b2, b3 (fp32)
+constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpW2 + kMlpHidden + kMlpHidden + 1;  // 6017
 
-template <bool MLP>
-__device__ __forceinline__ void mlp_load_weights(MlpShared<MLP>&, const KParams&, int) {}
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
-template <bool MLP, class RING>
-__device__ __forceinline__ float mlp_curvature(MlpShared<MLP>&, const KParams&, RING&, int, int, int, int, bool,
-                                               float, float fd) {
-    return fd;
+// GELU, tanh form, and its derivative.
+__device__ __forceinline__ float gelu(float x) {
+    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    return 0.5f * x * (1.0f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    const float t = tanhf(u);
+    return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+// Feature f of the 3x3x3 stencil sits at offset (f%3-1, (f/3)%3-1, f/9-1).
+// Normalised stencil: feature 13 (the centre) is 2*phi_c - 1, the others are
+// (phi_f - phi_c) / (|g| + 1e-4) with g the central-difference gradient of
+// phi at the centre, so the inputs are O(1) and the planar part of the
+// profile only sets their scale.  Inputs enter the matrix cores as a
+// hi + lo pair of BF16 numbers (about 16 significant bits): the curvature is
+// a percent-level second-order feature of the stencil that one BF16 rounding
+// would erase.
+__device__ __forceinline__ int feat_dx(int f) { return f % 3 - 1; }
+__device__ __forceinline__ int feat_dy(int f) { return (f / 3) % 3 - 1; }
+__device__ __forceinline__ int feat_dz(int f) { return f / 9 - 1; }
+constexpr int kCentre = 13;
+
+// inv = 1 / (|g| + 1e-4), with g from the six face neighbours
+// (features 12/14, 10/16, 4/22).
+__device__ __forceinline__ float feat_scale(float xm, float xp, float ym, float yp, float zm, float zp) {
+    const float gx = 0.5f * (xp - xm), gy = 0.5f * (yp - ym), gz = 0.5f * (zp - zm);
+    return 1.0f / (sqrtf(gx * gx + gy * gy + gz * gz) + 1e-4f);
+}
+
+// ---------------------------------------------------------------------------
+// inference: 16 cells per wavefront job
+// ---------------------------------------------------------------------------
+struct MlpWeightsLds {
+    __bf16 w[kMlpWords];
+    float b[kMlpBias];
+};
+
+// Network output for the 16 cells of one job: lane i + 16g contributes
+// features 8g..8g+7 of cell i as xh + xl.  Returns the prediction in every
+// lane of column i (lanes i, i+16, i+32, i+48).
+__device__ __forceinline__ float mlp_eval16(const MlpWeightsLds& s, bf16x8 xh, bf16x8 xl) {
+    const int lane = threadIdx.x & 63, i = lane & 15, g = lane >> 4;
+    const __bf16* W1 = s.w;
+    const __bf16* W2 = s.w + kMlpW1;
+    const __bf16* w3 = s.w + kMlpW1 + kMlpW2;
+    f32x4 d1[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        bf16x8 a;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) a[r] = W1[(16 * m + i) * kMlpInPad + 8 * g + r];
+        d1[m] = mfma32(a, xl, mfma32(a, xh, f32x4{0.0f, 0.0f, 0.0f, 0.0f}));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) d1[m][r] = gelu(d1[m][r] + s.b[16 * m + 4 * g + r]);
+    }
+    float y = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            bf16x8 a, b;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                a[r] = W2[(16 * m + i) * kMlpHidden + 32 * ks + 4 * g + r];
+                a[4 + r] = W2[(16 * m + i) * kMlpHidden + 32 * ks + 16 + 4 * g + r];
+                b[r] = (__bf16)d1[2 * ks][r];
+                b[4 + r] = (__bf16)d1[2 * ks + 1][r];
+            }
+            acc = mfma32(a, b, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int h = 16 * m + 4 * g + r;
+            y += (float)w3[h] * gelu(acc[r] + s.b[kMlpHidden + h]);
+        }
+    }
+    y += __shfl_xor(y, 16, 64);
+    y += __shfl_xor(y, 32, 64);
+    return y + s.b[2 * kMlpHidden];
 }
 
 }  // namespace pf

@@ -10,6 +10,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <random>
+#include <algorithm>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -32,6 +35,7 @@ struct pf_ctx {
     double* dout = nullptr;
     int nred = 0;
     float* gather_buf = nullptr;
+    float* kappa = nullptr;
     uint16_t* mlp_w = nullptr;
     float* mlp_b = nullptr;
     bool has_mlp = false;
@@ -69,6 +73,28 @@ uint16_t f2bf(float v) {
     const uint32_t lsb = (x >> 16) & 1u;
     x += 0x7fffu + lsb;
     return (uint16_t)(x >> 16);
+}
+
+// Device copy of the closure parameters: BF16 W1 padded to [64][32], W2
+// [64][64], w3 [64]; FP32 b1, b2, b3.
+void pack_mlp(const float* params, std::vector<uint16_t>& w, std::vector<float>& b) {
+    w.assign(64 * 32 + 64 * 64 + 64, 0);
+    b.assign(2 * 64 + 1, 0.0f);
+    const float* W1 = params;
+    const float* B1 = W1 + 64 * 27;
+    const float* W2 = B1 + 64;
+    const float* B2 = W2 + 64 * 64;
+    const float* W3 = B2 + 64;
+    const float* B3 = W3 + 64;
+    for (int o = 0; o < 64; ++o)
+        for (int i = 0; i < 27; ++i) w[o * 32 + i] = f2bf(W1[o * 27 + i]);
+    for (int i = 0; i < 64 * 64; ++i) w[64 * 32 + i] = f2bf(W2[i]);
+    for (int i = 0; i < 64; ++i) w[64 * 32 + 64 * 64 + i] = f2bf(W3[i]);
+    for (int i = 0; i < 64; ++i) {
+        b[i] = B1[i];
+        b[64 + i] = B2[i];
+    }
+    b[128] = B3[0];
 }
 
 // Physical and relaxation constants, evaluated exactly as the reference's
@@ -116,6 +142,8 @@ void fill_constants(pf_ctx* c) {
     k.seed = g.seed;
     k.mlp_lo = g.mlp_lo;
     k.mlp_hi = g.mlp_hi;
+    k.mlp_kmin = g.mlp_kmin;
+    k.mlp_kmax = g.mlp_kmax;
 }
 
 // z cells marched per workgroup.  Measured on MI355X (D3Q27 jet
@@ -172,6 +200,8 @@ int pf_config_default(pf_config* cfg, int32_t flow_case, int32_t velocity_set) {
     cfg->mlp_lo = 0.05f;
     cfg->mlp_hi = 0.95f;
     cfg->slot_pad = -1;
+    cfg->mlp_kmin = -0.6f;
+    cfg->mlp_kmax = -0.06f;
     cfg->axis_x = 2.0f;  // src/initialConditions.cu:74 halves the x offset
     cfg->axis_y = 1.0f;
     cfg->axis_z = 1.0f;
@@ -269,6 +299,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     c->nred = 1024;
     ok = ok && alloc((void**)&c->partials, (size_t)c->nred * 6 * sizeof(double));
     ok = ok && alloc((void**)&c->dout, 8 * sizeof(double));
+    if (g.closure == PF_CLOSURE_MLP) ok = ok && alloc((void**)&c->kappa, N * sizeof(float));
     if (!ok) {
         pf_destroy(c);
         return PF_ERR_OOM;
@@ -279,6 +310,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     k.g = c->g;
     for (int i = 0; i < pf::kNumFields; ++i) k.fld[i] = c->fld[i];
     k.bcp = c->bcp;
+    k.kappa = c->kappa;
     k.nx = g.nx;
     k.ny = g.ny;
     k.nz = g.nz;
@@ -304,6 +336,7 @@ void pf_destroy(pf_ctx* c) {
     (void)hipFree(c->partials);
     (void)hipFree(c->dout);
     (void)hipFree(c->gather_buf);
+    (void)hipFree(c->kappa);
     (void)hipFree(c->mlp_w);
     (void)hipFree(c->mlp_b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -377,6 +410,7 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
         const int variant = c->cfg.closure == PF_CLOSURE_MLP ? 2 : c->cfg.kernel;
         if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
         PF_HIP(c, pf::launch_phase_sum(c->kp, c->Q, c->jet, c->half, c->parity, c->stream));
+        if (variant == 2) PF_HIP(c, pf::launch_closure(c->kp, c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[1], c->stream));
         PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[2], c->stream));
@@ -460,16 +494,17 @@ int pf_set_closure(pf_ctx* c, const float* W1, const float* b1, const float* W2,
                    const float* b3) {
     if (!c || !W1 || !b1 || !W2 || !b2 || !W3 || !b3) return PF_ERR_CONFIG;
     PF_HIP(c, hipSetDevice(c->dev));
-    const int H = pf::kMlpHidden, I = pf::kMlpIn;
-    std::vector<uint16_t> w((size_t)H * I + (size_t)H * H + H);
+    std::vector<float> params(PF_MLP_PARAMS);
     size_t o = 0;
-    for (int i = 0; i < H * I; ++i) w[o++] = f2bf(W1[i]);
-    for (int i = 0; i < H * H; ++i) w[o++] = f2bf(W2[i]);
-    for (int i = 0; i < H; ++i) w[o++] = f2bf(W3[i]);
-    std::vector<float> b(2 * H + 1);
-    for (int i = 0; i < H; ++i) b[i] = b1[i];
-    for (int i = 0; i < H; ++i) b[H + i] = b2[i];
-    b[2 * H] = b3[0];
+    for (int i = 0; i < 64 * 27; ++i) params[o++] = W1[i];
+    for (int i = 0; i < 64; ++i) params[o++] = b1[i];
+    for (int i = 0; i < 64 * 64; ++i) params[o++] = W2[i];
+    for (int i = 0; i < 64; ++i) params[o++] = b2[i];
+    for (int i = 0; i < 64; ++i) params[o++] = W3[i];
+    params[o++] = b3[0];
+    std::vector<uint16_t> w;
+    std::vector<float> b;
+    pack_mlp(params.data(), w, b);
     if (!c->mlp_w) {
         if (hipMalloc((void**)&c->mlp_w, w.size() * sizeof(uint16_t)) != hipSuccess ||
             hipMalloc((void**)&c->mlp_b, b.size() * sizeof(float)) != hipSuccess)
@@ -517,6 +552,262 @@ int pf_time_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, float out_ms[4]) {
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return PF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned closure: training on the device
+// ---------------------------------------------------------------------------
+struct pf_mlp {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    float *master = nullptr, *m1 = nullptr, *m2 = nullptr, *bias = nullptr, *parts = nullptr, *gsum = nullptr;
+    uint16_t* w16 = nullptr;
+    float* xs = nullptr;
+    float* ts = nullptr;
+    int* idx = nullptr;
+    uint8_t* sym = nullptr;
+    double* loss = nullptr;
+    size_t n = 0, cap_parts = 0;
+    int step = 0;
+};
+
+namespace {
+int mlp_fail(hipError_t e) { return e == hipSuccess ? PF_OK : PF_ERR_LAUNCH; }
+
+// The 48 symmetries of the cube (axis permutations x sign flips) acting on
+// the 3x3x3 stencil: perm[s][f] = source feature of output feature f.
+std::vector<uint8_t> cube_symmetries() {
+    std::vector<uint8_t> t;
+    int ax[3] = {0, 1, 2};
+    do {
+        for (int sg = 0; sg < 8; ++sg) {
+            for (int f = 0; f < 27; ++f) {
+                const int d[3] = {f % 3 - 1, (f / 3) % 3 - 1, f / 9 - 1};
+                int e[3];
+                for (int a = 0; a < 3; ++a) e[a] = ((sg >> a) & 1 ? -1 : 1) * d[ax[a]];
+                t.push_back((uint8_t)((e[2] + 1) * 9 + (e[1] + 1) * 3 + (e[0] + 1)));
+            }
+        }
+    } while (std::next_permutation(ax, ax + 3));
+    return t;
+}
+
+int mlp_upload_params(pf_mlp* m, const float* params) {
+    std::vector<uint16_t> w;
+    std::vector<float> b;
+    pack_mlp(params, w, b);
+    if (hipMemcpy(m->master, params, PF_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->w16, w.data(), w.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->bias, b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(m->m1, 0, PF_MLP_PARAMS * 4) != hipSuccess || hipMemset(m->m2, 0, PF_MLP_PARAMS * 4) != hipSuccess)
+        return PF_ERR_LAUNCH;
+    m->step = 0;
+    return PF_OK;
+}
+
+int mlp_ensure_parts(pf_mlp* m, size_t blocks) {
+    if (blocks <= m->cap_parts) return PF_OK;
+    (void)hipFree(m->parts);
+    m->parts = nullptr;
+    if (hipMalloc((void**)&m->parts, blocks * (PF_MLP_PARAMS + 1) * sizeof(float)) != hipSuccess) return PF_ERR_OOM;
+    m->cap_parts = blocks;
+    return PF_OK;
+}
+
+std::vector<float> pad_features(const float* x, size_t n) {
+    std::vector<float> out(n * 32, 0.0f);
+    for (size_t s = 0; s < n; ++s)
+        for (int f = 0; f < 27; ++f) out[s * 32 + f] = x[s * 27 + f];
+    return out;
+}
+}  // namespace
+
+int pf_mlp_create(int32_t device, pf_mlp** out) {
+    if (!out) return PF_ERR_CONFIG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return PF_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return PF_ERR_DEVICE;
+    pf_mlp* m = new (std::nothrow) pf_mlp();
+    if (!m) return PF_ERR_OOM;
+    m->dev = device;
+    bool ok = hipStreamCreate(&m->stream) == hipSuccess;
+    ok = ok && hipMalloc((void**)&m->master, PF_MLP_PARAMS * 4) == hipSuccess;
+    ok = ok && hipMalloc((void**)&m->m1, PF_MLP_PARAMS * 4) == hipSuccess;
+    ok = ok && hipMalloc((void**)&m->m2, PF_MLP_PARAMS * 4) == hipSuccess;
+    ok = ok && hipMalloc((void**)&m->gsum, PF_MLP_PARAMS * 4) == hipSuccess;
+    ok = ok && hipMalloc((void**)&m->w16, (64 * 32 + 64 * 64 + 64) * 2) == hipSuccess;
+    ok = ok && hipMalloc((void**)&m->bias, 129 * 4) == hipSuccess;
+    ok = ok && hipMalloc((void**)&m->loss, sizeof(double)) == hipSuccess;
+    const std::vector<uint8_t> sym = cube_symmetries();
+    ok = ok && hipMalloc((void**)&m->sym, sym.size()) == hipSuccess;
+    ok = ok && hipMemcpy(m->sym, sym.data(), sym.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        pf_mlp_destroy(m);
+        return PF_ERR_OOM;
+    }
+    // default initialisation: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in))
+    std::vector<float> p(PF_MLP_PARAMS, 0.0f);
+    std::mt19937 rng(1234);
+    auto fill = [&](size_t off, size_t count, int fan_in) {
+        std::uniform_real_distribution<float> u(-1.0f / std::sqrt((float)fan_in), 1.0f / std::sqrt((float)fan_in));
+        for (size_t i = 0; i < count; ++i) p[off + i] = u(rng);
+    };
+    fill(0, 64 * 27, 27);
+    fill(64 * 27 + 64, 64 * 64, 64);
+    fill(64 * 27 + 64 + 64 * 64 + 64, 64, 64);
+    if (mlp_upload_params(m, p.data())) {
+        pf_mlp_destroy(m);
+        return PF_ERR_LAUNCH;
+    }
+    *out = m;
+    return PF_OK;
+}
+
+void pf_mlp_destroy(pf_mlp* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->dev);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    for (void* p : {(void*)m->master, (void*)m->m1, (void*)m->m2, (void*)m->bias, (void*)m->parts, (void*)m->gsum,
+                    (void*)m->w16, (void*)m->xs, (void*)m->ts, (void*)m->idx, (void*)m->sym, (void*)m->loss})
+        (void)hipFree(p);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int pf_mlp_set_params(pf_mlp* m, const float* params) {
+    if (!m || !params) return PF_ERR_CONFIG;
+    if (hipSetDevice(m->dev) != hipSuccess) return PF_ERR_DEVICE;
+    return mlp_upload_params(m, params);
+}
+
+int pf_mlp_get_params(pf_mlp* m, float* params) {
+    if (!m || !params) return PF_ERR_CONFIG;
+    if (hipSetDevice(m->dev) != hipSuccess) return PF_ERR_DEVICE;
+    if (hipStreamSynchronize(m->stream) != hipSuccess) return PF_ERR_LAUNCH;
+    return mlp_fail(hipMemcpy(params, m->master, PF_MLP_PARAMS * 4, hipMemcpyDeviceToHost));
+}
+
+int pf_mlp_set_data(pf_mlp* m, const float* x, const float* t, size_t n) {
+    if (!m || !x || !t || n == 0 || n > ((size_t)1 << 31)) return PF_ERR_CONFIG;
+    if (hipSetDevice(m->dev) != hipSuccess) return PF_ERR_DEVICE;
+    (void)hipFree(m->xs);
+    (void)hipFree(m->ts);
+    (void)hipFree(m->idx);
+    m->xs = nullptr;
+    m->ts = nullptr;
+    m->idx = nullptr;
+    const std::vector<float> xb = pad_features(x, n);
+    if (hipMalloc((void**)&m->xs, xb.size() * 4) != hipSuccess || hipMalloc((void**)&m->ts, n * 4) != hipSuccess ||
+        hipMalloc((void**)&m->idx, n * 4) != hipSuccess)
+        return PF_ERR_OOM;
+    if (hipMemcpy(m->xs, xb.data(), xb.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->ts, t, n * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return PF_ERR_LAUNCH;
+    m->n = n;
+    return PF_OK;
+}
+
+int pf_mlp_train(pf_mlp* m, uint32_t epochs, uint32_t batch, float lr, uint32_t seed, int32_t augment,
+                 float* epoch_loss) {
+    if (!m || m->n == 0 || batch == 0 || batch % 64 != 0 || !(lr > 0)) return PF_ERR_CONFIG;
+    if (hipSetDevice(m->dev) != hipSuccess) return PF_ERR_DEVICE;
+    if (int rc = mlp_ensure_parts(m, batch / 64)) return rc;
+    std::vector<int> perm(m->n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937 rng(seed);
+    for (uint32_t e = 0; e < epochs; ++e) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        if (hipMemcpyAsync(m->idx, perm.data(), m->n * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+            hipMemsetAsync(m->loss, 0, sizeof(double), m->stream) != hipSuccess)
+            return PF_ERR_LAUNCH;
+        const uint32_t sym_seed = seed * 0x85EBCA6Bu ^ (e + 1) * 0xC2B2AE35u;
+        for (size_t s0 = 0; s0 < m->n; s0 += batch) {
+            const size_t count = std::min<size_t>(batch, m->n - s0);
+            const size_t blocks = (count + 63) / 64;
+            hipError_t err = pf::launch_mlp_grad(m->w16, m->bias, m->xs, m->ts, m->idx, augment ? m->sym : nullptr, 48,
+                                                 sym_seed, m->n, s0, count, 1.0f / (float)count, m->parts, m->stream);
+            if (err != hipSuccess) return PF_ERR_LAUNCH;
+            err = pf::launch_mlp_adam(m->parts, (int)blocks, m->master, m->m1, m->m2, m->w16, m->bias, lr, ++m->step,
+                                      1, nullptr, m->loss, m->stream);
+            if (err != hipSuccess) return PF_ERR_LAUNCH;
+        }
+        double l = 0.0;
+        if (hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+            hipStreamSynchronize(m->stream) != hipSuccess)
+            return PF_ERR_LAUNCH;
+        if (epoch_loss) epoch_loss[e] = (float)(l / (double)m->n);
+    }
+    return PF_OK;
+}
+
+int pf_mlp_gradients(pf_mlp* m, size_t n, float* grad, double* mse) {
+    if (!m || !grad || n == 0 || n > m->n) return PF_ERR_CONFIG;
+    if (hipSetDevice(m->dev) != hipSuccess) return PF_ERR_DEVICE;
+    const size_t blocks = (n + 63) / 64;
+    if (int rc = mlp_ensure_parts(m, blocks)) return rc;
+    if (hipMemsetAsync(m->loss, 0, sizeof(double), m->stream) != hipSuccess) return PF_ERR_LAUNCH;
+    if (pf::launch_mlp_grad(m->w16, m->bias, m->xs, m->ts, nullptr, nullptr, 1, 0, n, 0, n, 1.0f / (float)n,
+                            m->parts, m->stream) != hipSuccess)
+        return PF_ERR_LAUNCH;
+    if (pf::launch_mlp_adam(m->parts, (int)blocks, m->master, m->m1, m->m2, m->w16, m->bias, 0.0f, 1, 0, m->gsum,
+                            m->loss, m->stream) != hipSuccess)
+        return PF_ERR_LAUNCH;
+    double l = 0.0;
+    if (hipMemcpyAsync(grad, m->gsum, PF_MLP_PARAMS * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipMemcpyAsync(&l, m->loss, sizeof(double), hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return PF_ERR_LAUNCH;
+    if (mse) *mse = l / (double)n;
+    return PF_OK;
+}
+
+int pf_mlp_predict(pf_mlp* m, const float* x, size_t n, float* y) {
+    if (!m || !x || !y || n == 0) return PF_ERR_CONFIG;
+    if (hipSetDevice(m->dev) != hipSuccess) return PF_ERR_DEVICE;
+    const std::vector<float> xb = pad_features(x, n);
+    float* dx = nullptr;
+    float* dy = nullptr;
+    int rc = PF_OK;
+    if (hipMalloc((void**)&dx, xb.size() * 4) != hipSuccess || hipMalloc((void**)&dy, n * 4) != hipSuccess) {
+        rc = PF_ERR_OOM;
+    } else if (hipMemcpyAsync(dx, xb.data(), xb.size() * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess ||
+               pf::launch_mlp_forward(m->w16, m->bias, dx, n, dy, m->stream) != hipSuccess ||
+               hipMemcpyAsync(y, dy, n * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+               hipStreamSynchronize(m->stream) != hipSuccess) {
+        rc = PF_ERR_LAUNCH;
+    }
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    return rc;
+}
+
+int pf_mfma_check(int32_t device, const float* a, const float* b, float* c) {
+    if (!a || !b || !c) return PF_ERR_CONFIG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return PF_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return PF_ERR_DEVICE;
+    std::vector<uint16_t> ab(16 * 32), bb(32 * 16);
+    for (int i = 0; i < 16 * 32; ++i) {
+        ab[i] = f2bf(a[i]);
+        bb[i] = f2bf(b[i]);
+    }
+    uint16_t *da = nullptr, *db = nullptr;
+    float* dc = nullptr;
+    int rc = PF_OK;
+    if (hipMalloc((void**)&da, 1024) != hipSuccess || hipMalloc((void**)&db, 1024) != hipSuccess ||
+        hipMalloc((void**)&dc, 1024) != hipSuccess) {
+        rc = PF_ERR_OOM;
+    } else if (hipMemcpy(da, ab.data(), 1024, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(db, bb.data(), 1024, hipMemcpyHostToDevice) != hipSuccess ||
+               pf::launch_mfma_layout(da, db, dc, nullptr) != hipSuccess ||
+               hipMemcpy(c, dc, 1024, hipMemcpyDeviceToHost) != hipSuccess) {
+        rc = PF_ERR_LAUNCH;
+    }
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dc);
+    return rc;
 }
 
 int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double* gbs) {

@@ -29,7 +29,6 @@
 
 #include "kparams.hpp"
 #include "lattice.hpp"
-#include "mlp_device.hpp"
 
 namespace pf {
 
@@ -278,7 +277,6 @@ template <int Q, bool JET, bool HALF, int PAR, bool MLP, bool LIT, bool SAVE>
 __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     __shared__ float s_phi[5][kHY * kHX];
     __shared__ float s_n[3][4][kNY1 * kNX1];
-    __shared__ MlpShared<MLP> s_mlp;
     constexpr int NP = (Q - 1) / 2;
 
     const int tid = threadIdx.x;
@@ -293,8 +291,6 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     const int zs = 1 + bz * p.zchunk;
     const int ze = min(zs + p.zchunk, nz - 1);
     const float* __restrict__ phi_g = p.fld[10];
-
-    if constexpr (MLP) mlp_load_weights(s_mlp, p, tid);
 
     auto load_phi = [&](int pz) {
         float* dst = s_phi[(pz + 5) % 5];
@@ -375,17 +371,20 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             const float scz = iso_sum<Q, 2>([&](auto qc) { return nval(2, qc); });
             curvature = kAs2 * (scx + scy + scz);
         }
-        if constexpr (MLP) {
-            curvature = mlp_curvature(s_mlp, p, s_phi, z, tx, ty, tid, active, phi, curvature);
-        }
         if (!active) continue;
+        const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+        if constexpr (MLP) {
+            // learned closure at interface cells whose FD curvature lies in
+            // the trained range (evaluated by k_closure for this step)
+            if (phi > p.mlp_lo && phi < p.mlp_hi && curvature >= p.mlp_kmin && curvature <= p.mlp_kmax)
+                curvature = p.kappa[c];
+        }
 
         const float ind = s_n[z % 3][3][o];
         const float nrx = s_n[z % 3][0][o], nry = s_n[z % 3][1][o], nrz = s_n[z % 3][2][o];
         const float stCurv = -p.sigma * curvature * ind;
         const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
 
-        const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
         const Nbr nb = make_nbr<!JET>(x, y, z, nx, ny, nz);
 
         // ---- relaxation (src/lbm.cu:158-172)
@@ -812,11 +811,8 @@ static hipError_t step_v(const KParams& p, int parity, int variant, bool save, h
                     : step_t<Q, JET, HALF, false, true, false>(p, parity, s);
     }
     if (variant == 2) {
-        if constexpr (kMlpKernels) {
-            return save ? step_t<Q, JET, HALF, true, false, true>(p, parity, s)
-                        : step_t<Q, JET, HALF, true, false, false>(p, parity, s);
-        }
-        return hipErrorNotSupported;
+        return save ? step_t<Q, JET, HALF, true, false, true>(p, parity, s)
+                    : step_t<Q, JET, HALF, true, false, false>(p, parity, s);
     }
     return save ? step_t<Q, JET, HALF, false, false, true>(p, parity, s)
                 : step_t<Q, JET, HALF, false, false, false>(p, parity, s);

@@ -1,0 +1,127 @@
+"""The committed learned curvature closure against the finite-difference one.
+
+The finite-difference curvature is the reference's kappa = div(n)
+(src/phaseField.cuh:142-219); the closure replaces it at interface cells by
+the 27 -> 64 -> 64 -> 1 network (phasefieldlbm_amd/closure_weights.npz),
+trained for the reference's phase relaxation time tau_g = 1.  The exact
+answer on a sphere of radius R is kappa = -2/R at every interface cell.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import oracle.mlp_ref as ref
+from phasefieldlbm_amd import closure, presets
+
+
+def _relaxed_sphere(R, n, steps):
+    """Static droplet relaxed by the CPU oracle; returns phi, the interface
+    cells the closure replaces, and the FD curvature there."""
+    sigma = 1e-3
+    cfg = presets.static_droplet(n=n, radius=float(R), sigma=sigma, tau_g=closure.TAU_G)
+    o = oracle.Oracle(cfg, 32).init()
+    o.step(steps)
+    phi = o.field("phi")
+    ind = o.field("ind").astype(np.float64)
+    nv = np.stack([o.field(k) for k in ("normx", "normy", "normz")]).astype(np.float64)
+    fs = np.stack([o.field(k) for k in ("fsx", "fsy", "fsz")]).astype(np.float64)
+    o.close()
+    inner = np.zeros(phi.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    cells = np.flatnonzero((inner & (phi > cfg.mlp_lo) & (phi < cfg.mlp_hi) & (ind > 1e-3)).ravel())
+    n2 = (nv ** 2).sum(0).ravel()[cells]
+    k_fd = -(fs * nv).sum(0).ravel()[cells] / (sigma * ind.ravel()[cells] * n2)
+    return phi, cells, k_fd
+
+
+def _rel(k, R):
+    want = -2.0 / R
+    return (k.mean() - want) / abs(want), np.sqrt(np.mean((k - want) ** 2)) / abs(want)
+
+
+def test_weights_file():
+    p = closure.load()
+    assert p.shape == (6017,) and p.dtype == np.float32 and np.all(np.isfinite(p))
+
+
+def test_closest_point_projection():
+    """Exact distance and closest point on an ellipsoid: on the surface, the
+    offset is along the surface normal, and a brute-force surface sample is
+    never closer; on a sphere the label is -2/R everywhere."""
+    rng = np.random.default_rng(0)
+    axes = np.array([14.0, 9.0, 6.0])
+    p = rng.uniform(-20, 20, (300, 3))
+    x, d = closure.closest_point(axes, p)
+    np.testing.assert_allclose(((x / axes) ** 2).sum(-1), 1.0, atol=1e-12)
+    nrm = x / axes ** 2
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    off = p - x
+    cross = np.linalg.norm(np.cross(off, nrm), axis=1)
+    assert np.all(cross <= 1e-9 * (1 + np.linalg.norm(off, axis=1)))
+    u = rng.normal(size=(200000, 3))
+    s = u / np.linalg.norm(u, axis=1, keepdims=True) * axes
+    brute = np.array([np.sqrt(((s - q) ** 2).sum(1)).min() for q in p[:50]])
+    assert np.all(np.abs(d[:50]) <= brute + 1e-9)
+    assert np.all(np.sign(d) == np.sign(((p / axes) ** 2).sum(-1) - 1.0))
+    _, ks = closure.ellipsoid_fields(np.full(3, 10.0), np.eye(3), np.full(3, 31.5))
+    np.testing.assert_allclose(ks, -0.2, rtol=1e-6)
+
+
+def test_stencil_features_planar_profile():
+    """Normalised features of a planar tanh profile along x: the centre is
+    2 phi - 1, the y/z neighbours in the cell's own plane are zero, and the
+    features do not depend on the profile's amplitude scale."""
+    n = 12
+    x = np.arange(n, dtype=np.float64)
+    prof = 0.5 * (1 + np.tanh(-2 * (x - 5.3) / 4.0))
+    phi = np.broadcast_to(prof[None, None, :], (n, n, n)).astype(np.float32).copy()
+    cell = np.ravel_multi_index((6, 6, 5), phi.shape)
+    f = closure.stencils(phi, np.array([cell]))[0]
+    assert abs(f[13] - (2 * phi[6, 6, 5] - 1)) < 1e-6
+    same_plane = [fi for fi in range(27) if fi % 3 == 1 and fi != 13]
+    np.testing.assert_allclose(f[same_plane], 0.0, atol=1e-6)
+    g = 0.5 * (prof[6] - prof[4])
+    assert abs(f[14] - (prof[6] - prof[5]) / (abs(g) + 1e-4)) < 1e-4
+    f_half = closure.stencils(0.5 * phi, np.array([cell]))[0]
+    off = [fi for fi in range(27) if fi != 13]
+    np.testing.assert_allclose(f_half[off], f[off], rtol=2e-3, atol=1e-5)
+
+
+def test_committed_closure_beats_fd_pointwise():
+    """On a sphere of radius 8 relaxed by the oracle, the network's curvature
+    is within 2 % of -2/R on average and its rms error is at most a quarter
+    of the finite-difference one (measured: 0.8 % against 21 %)."""
+    phi, cells, k_fd = _relaxed_sphere(8, 40, 300)
+    assert cells.size > 1000
+    k_mlp = 0.25 * ref.forward(closure.load(), closure.stencils(phi, cells), bf=True)[0]
+    m_fd, r_fd = _rel(k_fd, 8)
+    m_mlp, r_mlp = _rel(k_mlp, 8)
+    assert abs(m_mlp) < 0.02, m_mlp
+    assert r_mlp < 0.25 * r_fd, (r_mlp, r_fd)
+
+
+@pytest.mark.gpu
+def test_closure_pointwise_gpu():
+    """Same comparison with the library's own relaxation and device network,
+    radii 6 to 20."""
+    rows = closure.curvature_error(closure.load(), radii=(6, 12, 20), steps=300)
+    for row in rows:
+        assert abs(row["mlp"]["mean_rel"]) < 0.02, row
+        assert row["mlp"]["rms_rel"] < 0.25 * row["fd"]["rms_rel"], row
+
+
+@pytest.mark.gpu
+def test_closure_laplace():
+    """Static droplets run to rest with the learned closure stay at rest and
+    obey Laplace's law.  At R = 8, where the finite-difference curvature is
+    least accurate, the pressure jump is closer to 2 sigma / R than with it
+    (measured: 0.5 % against 4.0 %); at R = 12 and 16 both are within a few
+    per cent (FD 0.3 % and 2.1 %, closure 1.9 % and 3.3 %).  The closure's
+    residual pointwise noise (about 1 % of kappa) raises the spurious
+    velocities from about 1e-4 to about 1e-3."""
+    rows = closure.laplace(closure.load(), radii=(8, 12, 16), steps=3000)
+    by_r = {r["R"]: r for r in rows}
+    assert by_r[8]["mlp"]["rel_err"] < by_r[8]["fd"]["rel_err"], by_r[8]
+    for r in rows:
+        assert r["mlp"]["rel_err"] < 0.05, r
+        assert r["mlp"]["spurious_umax"] < 5e-3, r

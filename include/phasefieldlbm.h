@@ -170,17 +170,21 @@ int pf_phase_mass(pf_ctx *ctx, double *mass);
 
 /* Learned-closure weights, fp32 host arrays in row-major [out][in]:
  * W1[64*27], b1[64], W2[64*64], b2[64], W3[64], b3[1].  Stored on the
- * device as BF16 weights and FP32 biases.  The network maps the 3x3x3 phi
- * stencil (input 2*phi - 1, feature index (dz+1)*9 + (dy+1)*3 + (dx+1)) to
- * 4 x curvature.  No counterpart in the reference. */
+ * device as BF16 weights and FP32 biases.  The network maps the normalised
+ * 3x3x3 phi stencil to 4 x curvature: feature (dz+1)*9 + (dy+1)*3 + (dx+1)
+ * is (phi(x+d) - phi(x)) / (|g| + 1e-4), g the central-difference gradient
+ * of phi at x, except the centre (feature 13), which is 2*phi(x) - 1.  The
+ * features enter the matrix cores as hi + lo BF16 pairs.  No counterpart in
+ * the reference. */
 int pf_set_closure(pf_ctx *ctx, const float *W1, const float *b1, const float *W2,
                    const float *b2, const float *W3, const float *b3);
 
 /*
  * Training of the learned closure on the matrix cores (no counterpart in the
  * reference).  Parameters are one FP32 vector of PF_MLP_PARAMS values:
- * W1[64][27] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.  Samples are 27
- * stencil features (2*phi - 1) and a target (4 x curvature).
+ * W1[64][27] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.  Samples are the
+ * 27 stencil features described at pf_set_closure and a target
+ * (4 x curvature).
  */
 #define PF_MLP_PARAMS 6017
 typedef struct pf_mlp pf_mlp;
@@ -188,7 +192,7 @@ int pf_mlp_create(int32_t device, pf_mlp **out);
 void pf_mlp_destroy(pf_mlp *m);
 int pf_mlp_set_params(pf_mlp *m, const float *params);
 int pf_mlp_get_params(pf_mlp *m, float *params);
-/* Copy n samples (x: n*27 floats, stored as BF16; t: n floats) to the device. */
+/* Copy n samples (x: n*27 floats, kept in FP32; t: n floats) to the device. */
 int pf_mlp_set_data(pf_mlp *m, const float *x, const float *t, size_t n);
 /* Adam (beta 0.9/0.999) on the mean squared error over minibatches of
  * `batch` samples (a multiple of 64), shuffled each epoch from `seed`; with
@@ -207,8 +211,9 @@ int pf_mlp_predict(pf_mlp *m, const float *x, size_t n, float *y);
 int pf_mfma_check(int32_t device, const float *a, const float *b, float *c);
 
 /* Time `nsteps` steps (flags as pf_step) with HIP events on the context's
- * stream.  out_ms[0] = total, out_ms[1] = phase-sum kernel total,
- * out_ms[2] = fused collide-stream kernel total, out_ms[3] = boundary total.
+ * stream.  out_ms[0] = total, out_ms[1] = phase-sum kernel total (with the
+ * learned-closure pass when it is enabled), out_ms[2] = fused
+ * collide-stream kernel total, out_ms[3] = boundary total.
  * Per-kernel times are measured in a second, separately-evented pass. */
 int pf_time_steps(pf_ctx *ctx, uint64_t t0, uint32_t nsteps, float out_ms[4]);
 
@@ -217,6 +222,35 @@ int pf_time_steps(pf_ctx *ctx, uint64_t t0, uint32_t nsteps, float out_ms[4]);
  * / time.  No counterpart in the reference; used to state the measured peak
  * beside the roofline and to calibrate memory-traffic counters. */
 int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double *gbs);
+
+/*
+ * Output.  Formats are a bit mask: PF_OUT_BIN writes <dir>/<name><step6>.bin,
+ * raw FP32 of nx*ny*nz values (PostProcess::bin,
+ * src/postProcess/PostProcess.cuh:43-97); PF_OUT_VTI writes
+ * <dir>/step_<step6>.vti, VTK XML ImageData with raw appended FP32 point
+ * arrays, origin 0 and unit spacing (PostProcess::vti, :99-261); PF_OUT_VTR
+ * writes <dir>/step_<step6>.vtr, the same arrays on a VTK XML
+ * RectilinearGrid with coordinates 0..n-1.  <step6> is the step number
+ * zero-padded to six digits.  Names are those of pf_field_name.  The
+ * directory must exist.
+ */
+enum pf_output { PF_OUT_BIN = 1, PF_OUT_VTI = 2, PF_OUT_VTR = 4 };
+/* Fields `fields[0..nfields)` of the context as pf_get_field returns them;
+ * the main loop's output step (src/main.cu:178-185). */
+int pf_write_fields(pf_ctx *ctx, const char *dir, uint64_t step, const int32_t *fields, int32_t nfields,
+                    int32_t formats);
+/* One .vti (format PF_OUT_VTI) or .vtr (PF_OUT_VTR) file at `path` from
+ * host arrays of nx*ny*nz floats; needs no device. */
+int pf_write_vtk(const char *path, int32_t format, int32_t nx, int32_t ny, int32_t nz, const char *const *names,
+                 const float *const *arrays, int32_t narrays);
+/* Name of field `id` as the reference's output names it ("rho", "ux", ...,
+ * src/main.cu:57-76); NULL for an unknown id. */
+const char *pf_field_name(int32_t id);
+/* The configuration a context was created with (device resolved). */
+int pf_get_config(const pf_ctx *ctx, pf_config *out);
+/* Marketing name and gfx architecture of a device, "name (gfxNNN)", for the
+ * driver's banner (setDeviceFromEnv, src/functions/hostFunctions.cuh:117-150). */
+int pf_device_name(int32_t device, char *buf, size_t n);
 
 /* Bytes of device memory held by the context. */
 size_t pf_device_bytes(const pf_ctx *ctx);

@@ -23,7 +23,7 @@ from ._abi import (CLOSURE_FD, CLOSURE_MLP, D3Q19, D3Q27, DROPLET, FIELDS, JET, 
 __all__ = ["Config", "Simulation", "PfError", "presets", "FIELDS", "D3Q19", "D3Q27", "JET", "DROPLET",
            "STORE_FP32", "STORE_FP16", "NOISE_GAUSSIAN_XY", "NOISE_UNIFORM_Z", "NOISE_NONE",
            "CLOSURE_FD", "CLOSURE_MLP", "KERNEL_PAIR", "KERNEL_LITERAL", "library_path", "copy_bandwidth", "MLP",
-           "split_params", "mfma_check"]
+           "split_params", "mfma_check", "vtk"]
 
 
 def library_path() -> str:
@@ -220,13 +220,25 @@ class Simulation:
     def device_bytes(self) -> int:
         return int(self._lib.pf_device_bytes(self._h))
 
+    def write(self, directory: str, step: int, fields=("rho", "uz"), formats=("bin", "vtr")):
+        """Write fields as <name><step6>.bin, step_<step6>.vti and/or
+        step_<step6>.vtr in an existing directory (the reference's output
+        step, src/main.cu:178-185 and src/postProcess/PostProcess.cuh)."""
+        ids = (ctypes.c_int32 * len(fields))(*[_abi.FIELD_ID[f] for f in fields])
+        mask = 0
+        for f in formats:
+            mask |= {"bin": _abi.OUT_BIN, "vti": _abi.OUT_VTI, "vtr": _abi.OUT_VTR}[f]
+        self._check(self._lib.pf_write_fields(self._h, str(directory).encode(), int(step), ids, len(fields), mask),
+                    "pf_write_fields")
+
 
 class MLP:
     """The learned curvature closure's network on the device (pf_mlp).
 
     Parameters are one FP32 vector of 6017 values, W1[64][27] | b1 | W2[64][64]
-    | b2 | w3[64] | b3; samples are 27 stencil features (2*phi - 1, feature
-    (dz+1)*9 + (dy+1)*3 + (dx+1)) and a target 4*curvature.
+    | b2 | w3[64] | b3; samples are the 27 normalised stencil features of
+    ``closure.stencils`` (feature (dz+1)*9 + (dy+1)*3 + (dx+1)) and a target
+    4*curvature.
     """
 
     NPARAMS = _abi.MLP_PARAMS
@@ -311,4 +323,4 @@ def mfma_check(a, b, device: int = 0) -> np.ndarray:
     return c
 
 
-from . import presets  # noqa: E402
+from . import presets, vtk  # noqa: E402

@@ -20,6 +20,7 @@ NOISE_GAUSSIAN_XY, NOISE_UNIFORM_Z, NOISE_NONE = 0, 1, 2
 CLOSURE_FD, CLOSURE_MLP = 0, 1
 KERNEL_PAIR, KERNEL_LITERAL = 0, 1
 STEP_SAVE_LAST = 1
+OUT_BIN, OUT_VTI, OUT_VTR = 1, 2, 4
 MLP_PARAMS = 6017
 
 # host::FieldID order (src/fileIO/fields.cuh:33-68)
@@ -94,6 +95,13 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_double)]),
     "pf_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "pf_mfma_check": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pf_write_fields": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_void_p,
+                                       ctypes.c_int32, ctypes.c_int32]),
+    "pf_write_vtk": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32]),
+    "pf_field_name": (ctypes.c_char_p, [ctypes.c_int32]),
+    "pf_get_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PfConfig)]),
+    "pf_device_name": (ctypes.c_int, [ctypes.c_int32, ctypes.c_char_p, ctypes.c_size_t]),
     "pf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "pf_status_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

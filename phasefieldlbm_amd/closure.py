@@ -42,6 +42,9 @@ from . import CLOSURE_FD, CLOSURE_MLP, D3Q19, MLP, Simulation, presets, split_pa
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "closure_weights.npz")
+# The same parameters as 6017 little-endian float32 values, for the C++
+# driver (pfsim --closure).
+WEIGHTS_RAW = os.path.join(HERE, "closure_weights.f32")
 BOX = 64
 W = 4.0
 # The reference's phase relaxation time (src/constants.cuh:105, 134).  It sets
@@ -237,6 +240,10 @@ def save(params, path=WEIGHTS, **meta):
              **{k: np.asarray(v) for k, v in meta.items()})
 
 
+def save_raw(params, path=WEIGHTS_RAW):
+    np.asarray(params, dtype="<f4").tofile(path)
+
+
 def load(path=WEIGHTS):
     with np.load(path, allow_pickle=False) as z:
         return z["params"].astype(np.float32)
@@ -331,6 +338,7 @@ def main(argv=None):
     val_rmse = float(np.sqrt(np.mean((pred - T[val]) ** 2)))
     base_rmse = float(np.sqrt(np.mean((T[val] - T[tr].mean()) ** 2)))
     save(params, args.out, losses=losses, val_rmse=val_rmse, samples=T.size)
+    save_raw(params, os.path.splitext(args.out)[0] + ".f32")
     print(json.dumps({"samples": int(T.size), "train_mse": float(losses[-1]), "val_rmse": val_rmse,
                       "val_rmse_constant": base_rmse}), flush=True)
     print(json.dumps({"curvature": curvature_error(params)}), flush=True)

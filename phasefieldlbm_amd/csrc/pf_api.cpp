@@ -18,6 +18,7 @@
 
 #include "../../include/phasefieldlbm.h"
 #include "kparams.hpp"
+#include "pf_io.hpp"
 
 struct pf_ctx {
     pf_config cfg{};
@@ -519,6 +520,60 @@ int pf_set_closure(pf_ctx* c, const float* W1, const float* b1, const float* W2,
     return PF_OK;
 }
 
+namespace {
+const char* const kFieldNames[PF_NUM_FIELDS] = {"rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz",
+                                                 "pyz", "phi", "normx", "normy", "normz", "ind", "fsx", "fsy", "fsz"};
+}
+
+const char* pf_field_name(int32_t id) { return id >= 0 && id < PF_NUM_FIELDS ? kFieldNames[id] : nullptr; }
+
+int pf_get_config(const pf_ctx* c, pf_config* out) {
+    if (!c || !out) return PF_ERR_CONFIG;
+    *out = c->cfg;
+    out->device = c->dev;
+    return PF_OK;
+}
+
+int pf_device_name(int32_t device, char* buf, size_t n) {
+    if (!buf || n == 0) return PF_ERR_CONFIG;
+    hipDeviceProp_t prop{};
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PF_ERR_DEVICE;
+    std::snprintf(buf, n, "%s (%s)", prop.name, prop.gcnArchName);
+    return PF_OK;
+}
+
+int pf_write_fields(pf_ctx* c, const char* dir, uint64_t step, const int32_t* fields, int32_t nfields,
+                    int32_t formats) {
+    if (!c || !dir || !fields || nfields < 1) return PF_ERR_CONFIG;
+    if (formats <= 0 || (formats & ~(PF_OUT_BIN | PF_OUT_VTI | PF_OUT_VTR))) return fail(c, PF_ERR_CONFIG, "bad formats");
+    std::vector<std::string> names;
+    for (int32_t i = 0; i < nfields; ++i) {
+        if (!pf_field_name(fields[i])) return fail(c, PF_ERR_CONFIG, "bad field id");
+        names.emplace_back(pf_field_name(fields[i]));
+    }
+    const std::string sfx = pfio::step_suffix(step);
+    pfio::VtkWriter vti, vtr;
+    const int nx = c->cfg.nx, ny = c->cfg.ny, nz = c->cfg.nz;
+    if ((formats & PF_OUT_VTI) && !vti.open(pfio::join(dir, "step_" + sfx + ".vti"), pfio::kVti, nx, ny, nz, names))
+        return fail(c, PF_ERR_IO, std::string("cannot write ") + pfio::join(dir, "step_" + sfx + ".vti"));
+    if ((formats & PF_OUT_VTR) && !vtr.open(pfio::join(dir, "step_" + sfx + ".vtr"), pfio::kVtr, nx, ny, nz, names))
+        return fail(c, PF_ERR_IO, std::string("cannot write ") + pfio::join(dir, "step_" + sfx + ".vtr"));
+    std::vector<float> h(c->N);
+    for (int32_t i = 0; i < nfields; ++i) {
+        const int rc = pf_get_field(c, fields[i], h.data(), c->N);
+        if (rc) return rc;
+        if (formats & PF_OUT_BIN) {
+            const std::string path = pfio::join(dir, names[(size_t)i] + sfx + ".bin");
+            if (!pfio::write_bin(path, h.data(), c->N)) return fail(c, PF_ERR_IO, "cannot write " + path);
+        }
+        if ((formats & PF_OUT_VTI) && !vti.put(h.data())) return fail(c, PF_ERR_IO, "vti write failed");
+        if ((formats & PF_OUT_VTR) && !vtr.put(h.data())) return fail(c, PF_ERR_IO, "vtr write failed");
+    }
+    if ((formats & PF_OUT_VTI) && !vti.close()) return fail(c, PF_ERR_IO, "vti write failed");
+    if ((formats & PF_OUT_VTR) && !vtr.close()) return fail(c, PF_ERR_IO, "vtr write failed");
+    return PF_OK;
+}
+
 int pf_time_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, float out_ms[4]) {
     if (!c || !out_ms || nsteps == 0) return PF_ERR_CONFIG;
     if (!c->inited) return fail(c, PF_ERR_STATE, "timing before pf_init");
@@ -571,6 +626,7 @@ struct pf_mlp {
     int step = 0;
 };
 
+extern "C++" {
 namespace {
 int mlp_fail(hipError_t e) { return e == hipSuccess ? PF_OK : PF_ERR_LAUNCH; }
 
@@ -621,6 +677,7 @@ std::vector<float> pad_features(const float* x, size_t n) {
     return out;
 }
 }  // namespace
+}  // extern "C++"
 
 int pf_mlp_create(int32_t device, pf_mlp** out) {
     if (!out) return PF_ERR_CONFIG;

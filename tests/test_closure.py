@@ -125,3 +125,9 @@ def test_closure_laplace():
     for r in rows:
         assert r["mlp"]["rel_err"] < 0.05, r
         assert r["mlp"]["spurious_umax"] < 5e-3, r
+
+
+def test_raw_weights_match():
+    """closure_weights.f32 (read by the C++ driver) holds the same parameters."""
+    raw = np.fromfile(closure.WEIGHTS_RAW, dtype="<f4")
+    np.testing.assert_array_equal(raw, closure.load())

@@ -113,7 +113,7 @@ typedef struct pf_config {
     int32_t zchunk;        /* z cells marched per workgroup; 0 = automatic */
     float mlp_lo, mlp_hi;  /* MLP applies where mlp_lo < phi < mlp_hi */
     int32_t kernel;        /* pf_kernel */
-    int32_t slot_pad;      /* cells of padding between population slots; <0 = automatic */
+    int32_t slot_pad;      /* elements of padding per population row (beyond Q*nx); <0 = automatic (0) */
     float mlp_kmin, mlp_kmax; /* MLP applies where the FD curvature lies in [kmin, kmax] */
 } pf_config;
 

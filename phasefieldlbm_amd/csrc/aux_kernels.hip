@@ -15,19 +15,24 @@
 
 namespace pf {
 
-template <bool HALF>
-__device__ __forceinline__ void st_pop(void* base, uint32_t N, int slot, uint32_t i, float v) {
-    if constexpr (HALF)
-        reinterpret_cast<_Float16*>(base)[(size_t)slot * N + i] = (_Float16)v;
-    else
-        reinterpret_cast<float*>(base)[(size_t)slot * N + i] = v;
+// Element index of population `slot` of cell (x, y, z) in an array with row
+// pitch `pitch` and plane pitch `plane` (the layout of kparams.hpp).
+__device__ __forceinline__ uint64_t pop_elem(uint64_t plane, uint32_t pitch, int nx, int slot, int x, int y, int z) {
+    return (uint64_t)z * plane + (uint64_t)y * pitch + (uint64_t)slot * (uint32_t)nx + (uint32_t)x;
 }
 template <bool HALF>
-__device__ __forceinline__ float ld_pop(const void* base, uint32_t N, int slot, uint32_t i) {
+__device__ __forceinline__ void st_pop(void* base, uint64_t e, float v) {
     if constexpr (HALF)
-        return (float)reinterpret_cast<const _Float16*>(base)[(size_t)slot * N + i];
+        reinterpret_cast<_Float16*>(base)[e] = (_Float16)v;
     else
-        return reinterpret_cast<const float*>(base)[(size_t)slot * N + i];
+        reinterpret_cast<float*>(base)[e] = v;
+}
+template <bool HALF>
+__device__ __forceinline__ float ld_pop(const void* base, uint64_t e) {
+    if constexpr (HALF)
+        return (float)reinterpret_cast<const _Float16*>(base)[e];
+    else
+        return reinterpret_cast<const float*>(base)[e];
 }
 
 __global__ __launch_bounds__(256) void k_init_fields(KParams p, int jet) {
@@ -55,15 +60,18 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
     const float rho = p.fld[0][i], ux = p.fld[1][i], uy = p.fld[2][i], uz = p.fld[3][i];
     const float phi = p.fld[10][i];
     const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
+    const int x = (int)(i % (uint32_t)p.nx);
+    const int y = (int)((i / (uint32_t)p.nx) % (uint32_t)p.ny);
+    const int z = (int)(i / ((uint32_t)p.nx * (uint32_t)p.ny));
     static_for<Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        st_pop<HALF>(p.f, p.sN, q, i, f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
+        st_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, p.nx, q, x, y, z), f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
     });
     static_for<7>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         float v = g_post<q>(phi, ux, uy, uz, 0.0f, 0.0f, 0.0f, 0.0f);
         if constexpr (HALF) v -= p.wphi0[q];
-        st_pop<HALF>(p.g, p.sN, q, i, v);
+        st_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, p.nx, q, x, y, z), v);
     });
 }
 
@@ -89,17 +97,17 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
     for (int q = 0; q < nq; ++q) {
         float v = 0.0f;
         if (interior) {
-            uint32_t src = i;
-            int slot = q;
+            int sx = x, sy = y, sz = z, slot = q;
             if constexpr (PAR == 1) {
                 slot = opp(q);
-                src = (uint32_t)xs[1 - kCX[q]] +
-                      (uint32_t)nx * ((uint32_t)ys[1 - kCY[q]] + (uint32_t)ny * (uint32_t)zs[1 - kCZ[q]]);
+                sx = xs[1 - kCX[q]];
+                sy = ys[1 - kCY[q]];
+                sz = zs[1 - kCZ[q]];
             }
             if (which == 0) {
-                v = ld_pop<HALF>(p.f, p.sN, slot, src);
+                v = ld_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, nx, slot, sx, sy, sz));
             } else {
-                v = ld_pop<HALF>(p.g, p.sN, slot, src);
+                v = ld_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, nx, slot, sx, sy, sz));
                 if constexpr (HALF) v += p.wphi0[q];
             }
         }
@@ -176,14 +184,14 @@ __global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials)
         const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp}, zs[3] = {zm, z, zp};
         double phi = 0.0;
         for (int q = 0; q < 7; ++q) {
-            uint32_t src = i;
-            int slot = q;
+            int sx = x, sy = y, sz = z, slot = q;
             if constexpr (PAR == 1) {
                 slot = opp(q);
-                src = (uint32_t)xs[1 - kCX[q]] +
-                      (uint32_t)nx * ((uint32_t)ys[1 - kCY[q]] + (uint32_t)ny * (uint32_t)zs[1 - kCZ[q]]);
+                sx = xs[1 - kCX[q]];
+                sy = ys[1 - kCY[q]];
+                sz = zs[1 - kCZ[q]];
             }
-            float gv = ld_pop<HALF>(p.g, p.sN, slot, src);
+            float gv = ld_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, nx, slot, sx, sy, sz));
             if constexpr (HALF) gv += p.wphi0[q];
             phi += (double)gv;
         }

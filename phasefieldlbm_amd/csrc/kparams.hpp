@@ -2,12 +2,17 @@
 // and the C-ABI layer.
 //
 // HBM layout (one context):
-//   f   : Q slots x N cells, FP32 or binary16, slot-major (SoA), the
-//         reference's f[q*N + idx] (src/functions/deviceFunctions.cuh:33-48)
-//         with the slot stride padded to sN >= N (a stride of a large power
-//         of two puts every slot of a cell on the same HBM channel), and
-//         addressed in place with the A-A pattern (see step kernels)
-//   g   : 7 slots x N cells, same storage type as f
+//   f   : Q population slots of every cell, FP32 or binary16, interleaved
+//         per (y, z) row: element (slot, x, y, z) sits at
+//         z*plane_f + y*pitch_f + slot*nx + x with pitch_f >= Q*nx and
+//         plane_f = pitch_f*ny.  One row holds the Q populations of its nx
+//         cells as Q contiguous runs, so the populations a workgroup touches
+//         together are one contiguous block per row instead of Q streams a
+//         whole grid apart (the reference's slot-major f[q*N + idx],
+//         src/functions/deviceFunctions.cuh:33-48, made timings depend on
+//         where the driver placed the physical pages).  Addressed in place
+//         with the A-A pattern (see the step kernels).
+//   g   : 7 slots, same storage type and layout (pitch_g >= 7*nx)
 //   fld : 18 FP32 macroscopic fields [N] in host::FieldID order; phi is
 //         live every step, the others are written on save steps only
 //   bcp : jet boundary planes, 2 planes x 13 values x nx*ny (rho, u, P, F at
@@ -52,8 +57,10 @@ struct KParams {
     float* fld[kNumFields];
     float* bcp;
     int nx, ny, nz;
-    uint32_t N;   // cells
-    uint32_t sN;  // population slot stride in elements (N plus padding)
+    uint32_t N;  // cells
+    uint32_t pitch_f, pitch_g;  // population row pitch in elements (y stride); slot stride is nx
+    uint64_t plane_f, plane_g;  // population plane pitch in elements (z stride) = pitch * ny
+    uint64_t elems_f, elems_g;  // population array sizes in elements = plane * nz
     int zchunk;
     int ntx, nty, ntz;  // tile counts
     int save;           // write every macroscopic field this step

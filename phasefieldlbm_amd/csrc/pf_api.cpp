@@ -25,7 +25,7 @@ struct pf_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
     size_t N = 0;
-    size_t sN = 0;  // population slot stride (elements)
+    size_t f_elems = 0, g_elems = 0;  // population array sizes (elements, row-interleaved layout)
     int Q = 19;
     bool jet = true, half = false;
     void* f = nullptr;
@@ -250,15 +250,18 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     c->Q = g.velocity_set;
     c->jet = g.flow_case == PF_JET;
     c->half = g.storage == PF_STORE_FP16;
-    // Slot stride: N rounded up to 256 cells plus `slot_pad` cells (default
-    // 1 KiB-worth of cells times an odd factor) so that successive slots of
-    // the same cell start at different HBM channel offsets.
-    const long pad = g.slot_pad >= 0 ? g.slot_pad : (c->half ? 1536 : 768);
-    c->sN = ((N + 255) / 256) * 256 + (size_t)pad;
-    if ((c->sN * 27 * (c->half ? 2 : 4)) >> 36) {  // keep every byte offset representable
+    // Row-interleaved population layout: row pitch S*nx plus `slot_pad`
+    // elements (default none), plane pitch = row pitch * ny.  In-plane byte
+    // offsets are 32-bit in the kernels.
+    const uint64_t pad = g.slot_pad >= 0 ? (uint64_t)g.slot_pad : 0;
+    const uint64_t pitch_f = (uint64_t)c->Q * g.nx + pad, pitch_g = (uint64_t)7 * g.nx + pad;
+    const uint64_t esize = c->half ? 2 : 4;
+    if (pitch_f * g.ny * esize >= ((uint64_t)1 << 32)) {
         delete c;
         return PF_ERR_CONFIG;
     }
+    c->f_elems = (size_t)(pitch_f * g.ny * g.nz);
+    c->g_elems = (size_t)(pitch_g * g.ny * g.nz);
 
     // device selection (src/functions/hostFunctions.cuh:117-150)
     int dev = g.device;
@@ -293,7 +296,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
         c->bytes += bytes;
         return true;
     };
-    bool ok = alloc(&c->f, (size_t)c->Q * c->sN * pop_bytes(c)) && alloc(&c->g, (size_t)7 * c->sN * pop_bytes(c));
+    bool ok = alloc(&c->f, c->f_elems * pop_bytes(c)) && alloc(&c->g, c->g_elems * pop_bytes(c));
     for (int i = 0; ok && i < pf::kNumFields; ++i) ok = alloc((void**)&c->fld[i], N * sizeof(float));
     const size_t plane = (size_t)g.nx * g.ny;
     ok = ok && alloc((void**)&c->bcp, 2 * pf::kPlaneVals * plane * sizeof(float));
@@ -316,7 +319,12 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     k.ny = g.ny;
     k.nz = g.nz;
     k.N = (uint32_t)N;
-    k.sN = (uint32_t)c->sN;
+    k.pitch_f = (uint32_t)pitch_f;
+    k.pitch_g = (uint32_t)pitch_g;
+    k.plane_f = pitch_f * (uint64_t)g.ny;
+    k.plane_g = pitch_g * (uint64_t)g.ny;
+    k.elems_f = c->f_elems;
+    k.elems_g = c->g_elems;
     k.zchunk = choose_zchunk(g);
     k.ntx = (g.nx + pf::kTX - 1) / pf::kTX;
     k.nty = (g.ny + pf::kTY - 1) / pf::kTY;
@@ -363,8 +371,8 @@ int pf_init(pf_ctx* c) {
     const pf_config& g = c->cfg;
     for (int i = 0; i < pf::kNumFields; ++i)
         PF_HIP(c, hipMemsetAsync(c->fld[i], 0, c->N * sizeof(float), c->stream));
-    PF_HIP(c, hipMemsetAsync(c->f, 0, (size_t)c->Q * c->sN * pop_bytes(c), c->stream));
-    PF_HIP(c, hipMemsetAsync(c->g, 0, (size_t)7 * c->sN * pop_bytes(c), c->stream));
+    PF_HIP(c, hipMemsetAsync(c->f, 0, c->f_elems * pop_bytes(c), c->stream));
+    PF_HIP(c, hipMemsetAsync(c->g, 0, c->g_elems * pop_bytes(c), c->stream));
     PF_HIP(c, pf::launch_init_fields(c->kp, c->jet, g.radius, g.axis_x, g.axis_y, g.axis_z, g.interface_width,
                                      c->stream));
     if (!c->jet) {

@@ -40,49 +40,90 @@ constexpr int kNY1 = kTY + 2;
 // ---------------------------------------------------------------------------
 // population storage
 // ---------------------------------------------------------------------------
-template <bool HALF>
-struct Pop;
 // Population stores are consumed only by the next step, so they carry the
 // non-temporal (streaming) hint (measured ~7% faster on MI355X).  Loads keep
 // the default policy: in the odd step neighbouring tiles re-read the lines at
 // their common edge, and streaming loads measured slower.
-template <class T>
-__device__ __forceinline__ T pop_load(const T* p) {
-    return *p;
+template <bool HALF>
+struct Pop;
+
+// Location of a cell in a population array: plane index z and the in-plane
+// element offset x + y*pitch.  z is wave-uniform in every kernel that
+// marches planes.
+struct Loc {
+    uint32_t z;
+    uint32_t xy;
+};
+
+// Raw buffer resource of plane z of a population array (gfx9 descriptor
+// word 3: 32-bit data format).  Every access is then one 32-bit vector
+// offset (x + y*pitch) plus a scalar offset (slot*nx) from a scalar base.
+// num_records stops at the end of the array: an offset beyond it would read
+// zero and drop the write rather than touch other memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint64_t plane_bytes,
+                                                           uint64_t total_bytes, uint32_t z) {
+    const uint64_t off = (uint64_t)z * plane_bytes;
+    const uint64_t rem = total_bytes > off ? total_bytes - off : 0;
+    const uint32_t nrec = rem > 0xffffffffull ? 0xffffffffu : (uint32_t)rem;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(reinterpret_cast<const char*>(base)) + off, (short)0,
+                                             (int)nrec, 0x00020000);
 }
-template <class T>
-__device__ __forceinline__ void pop_store(T* p, T v) {
+
 #ifdef PF_CACHED_STORE
-    *p = v;
+constexpr int kStoreAux = 0;
 #else
-    __builtin_nontemporal_store(v, p);
+constexpr int kStoreAux = 2;  // nt: streaming store
 #endif
-}
 
 template <>
 struct Pop<false> {
     using T = float;
-    static __device__ __forceinline__ float ld(const void* base, uint32_t N, int slot, uint32_t i) {
-        const char* b = reinterpret_cast<const char*>(base) + (size_t)slot * N * 4u;
-        return pop_load(reinterpret_cast<const float*>(b + (i << 2)));
+    static __device__ __forceinline__ float ld(const void* base, uint64_t plane, uint64_t total, uint32_t nx, int slot,
+                                               Loc l) {
+        const auto r = plane_rsrc(base, plane * 4u, total * 4u, l.z);
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, l.xy << 2, (uint32_t)slot * nx * 4u, 0));
     }
-    static __device__ __forceinline__ void st(void* base, uint32_t N, int slot, uint32_t i, float v) {
-        char* b = reinterpret_cast<char*>(base) + (size_t)slot * N * 4u;
-        pop_store(reinterpret_cast<float*>(b + (i << 2)), v);
+    static __device__ __forceinline__ void st(void* base, uint64_t plane, uint64_t total, uint32_t nx, int slot, Loc l,
+                                              float v) {
+        const auto r = plane_rsrc(base, plane * 4u, total * 4u, l.z);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, l.xy << 2, (uint32_t)slot * nx * 4u,
+                                              kStoreAux);
     }
 };
 template <>
 struct Pop<true> {
     using T = _Float16;
-    static __device__ __forceinline__ float ld(const void* base, uint32_t N, int slot, uint32_t i) {
-        const char* b = reinterpret_cast<const char*>(base) + (size_t)slot * N * 2u;
-        return (float)pop_load(reinterpret_cast<const _Float16*>(b + (i << 1)));
+    static __device__ __forceinline__ float ld(const void* base, uint64_t plane, uint64_t total, uint32_t nx, int slot,
+                                               Loc l) {
+        const auto r = plane_rsrc(base, plane * 2u, total * 2u, l.z);
+        const unsigned short h = __builtin_amdgcn_raw_buffer_load_b16(r, l.xy << 1, (uint32_t)slot * nx * 2u, 0);
+        return (float)__builtin_bit_cast(_Float16, h);
     }
-    static __device__ __forceinline__ void st(void* base, uint32_t N, int slot, uint32_t i, float v) {
-        char* b = reinterpret_cast<char*>(base) + (size_t)slot * N * 2u;
-        pop_store(reinterpret_cast<_Float16*>(b + (i << 1)), (_Float16)v);
+    static __device__ __forceinline__ void st(void* base, uint64_t plane, uint64_t total, uint32_t nx, int slot, Loc l,
+                                              float v) {
+        const auto r = plane_rsrc(base, plane * 2u, total * 2u, l.z);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)v), r, l.xy << 1,
+                                              (uint32_t)slot * nx * 2u, kStoreAux);
     }
 };
+
+// Accessors of the two population arrays of a KParams.
+template <bool HALF>
+__device__ __forceinline__ float ldf(const KParams& p, int slot, Loc l) {
+    return Pop<HALF>::ld(p.f, p.plane_f, p.elems_f, (uint32_t)p.nx, slot, l);
+}
+template <bool HALF>
+__device__ __forceinline__ void stf(const KParams& p, int slot, Loc l, float v) {
+    Pop<HALF>::st(p.f, p.plane_f, p.elems_f, (uint32_t)p.nx, slot, l, v);
+}
+template <bool HALF>
+__device__ __forceinline__ float ldg(const KParams& p, int slot, Loc l) {
+    return Pop<HALF>::ld(p.g, p.plane_g, p.elems_g, (uint32_t)p.nx, slot, l);
+}
+template <bool HALF>
+__device__ __forceinline__ void stg(const KParams& p, int slot, Loc l, float v) {
+    Pop<HALF>::st(p.g, p.plane_g, p.elems_g, (uint32_t)p.nx, slot, l, v);
+}
 
 // Reciprocal: IEEE division by default; the hardware reciprocal (1 ulp)
 // when PF_FAST_RCP is defined.
@@ -94,14 +135,17 @@ __device__ __forceinline__ float recip(float x) {
 #endif
 }
 
-// Neighbour offsets of one cell: xw[dx+1] + row[dy+1][dz+1].
+// Neighbours of one cell (ghost-shell periodicity applied): x, y*pitch of
+// each population array and z for offsets -1, 0, +1 along each axis.
 struct Nbr {
     uint32_t xw[3];
-    uint32_t row[3][3];
+    uint32_t yf[3], yg[3];
+    uint32_t zz[3];
 };
 
 template <bool WRAPZ>
-__device__ __forceinline__ Nbr make_nbr(int x, int y, int z, int nx, int ny, int nz) {
+__device__ __forceinline__ Nbr make_nbr(const KParams& p, int x, int y, int z) {
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
     Nbr n;
     const int xm = (x == 1) ? nx - 2 : x - 1, xp = (x == nx - 2) ? 1 : x + 1;
     const int ym = (y == 1) ? ny - 2 : y - 1, yp = (y == ny - 2) ? 1 : y + 1;
@@ -113,29 +157,40 @@ __device__ __forceinline__ Nbr make_nbr(int x, int y, int z, int nx, int ny, int
     n.xw[0] = (uint32_t)xm;
     n.xw[1] = (uint32_t)x;
     n.xw[2] = (uint32_t)xp;
-    const int yy[3] = {ym, y, yp}, zz[3] = {zm, z, zp};
+    const int yy[3] = {ym, y, yp};
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) n.row[a][b] = (uint32_t)nx * ((uint32_t)yy[a] + (uint32_t)ny * (uint32_t)zz[b]);
+    for (int a = 0; a < 3; ++a) {
+        n.yf[a] = p.pitch_f * (uint32_t)yy[a];
+        n.yg[a] = p.pitch_g * (uint32_t)yy[a];
+    }
+    n.zz[0] = (uint32_t)zm;
+    n.zz[1] = (uint32_t)z;
+    n.zz[2] = (uint32_t)zp;
     return n;
 }
 
+// The cell itself and its neighbour x + SIGN*c_q in the f and g arrays.
+__device__ __forceinline__ Loc self_f(const Nbr& n) { return {n.zz[1], n.xw[1] + n.yf[1]}; }
+__device__ __forceinline__ Loc self_g(const Nbr& n) { return {n.zz[1], n.xw[1] + n.yg[1]}; }
 template <int q, int SIGN>
-__device__ __forceinline__ uint32_t nb_idx(const Nbr& n) {
-    return n.xw[SIGN * kCX[q] + 1] + n.row[SIGN * kCY[q] + 1][SIGN * kCZ[q] + 1];
+__device__ __forceinline__ Loc nb_f(const Nbr& n) {
+    return {n.zz[SIGN * kCZ[q] + 1], n.xw[SIGN * kCX[q] + 1] + n.yf[SIGN * kCY[q] + 1]};
+}
+template <int q, int SIGN>
+__device__ __forceinline__ Loc nb_g(const Nbr& n) {
+    return {n.zz[SIGN * kCZ[q] + 1], n.xw[SIGN * kCX[q] + 1] + n.yg[SIGN * kCY[q] + 1]};
 }
 
 // Where the next step of parity PAR reads in_q of the cell with neighbour
-// table n (cell index c).
-template <int q, int PAR>
-__device__ __forceinline__ void in_slot(const Nbr& n, uint32_t c, int& slot, uint32_t& idx) {
+// table n, in the f (G = false) or g (G = true) array.
+template <int q, int PAR, bool G>
+__device__ __forceinline__ void in_slot(const Nbr& n, int& slot, Loc& l) {
     if constexpr (PAR == 0) {
         slot = q;
-        idx = c;
+        l = G ? self_g(n) : self_f(n);
     } else {
         slot = opp(q);
-        idx = nb_idx<q, -1>(n);
+        l = G ? nb_g<q, -1>(n) : nb_f<q, -1>(n);
     }
 }
 
@@ -155,14 +210,13 @@ __global__ __launch_bounds__(kThreads) void k_phase_sum(KParams p) {
 #pragma unroll
     for (int k = 0; k < kPsZ; ++k) {
         const int z = min(z0 + k, p.nz - 2);  // clamped duplicates are not stored
-        const uint32_t c = (uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z);
-        const Nbr n = make_nbr<!JET>(x, y, z, p.nx, p.ny, p.nz);
+        const Nbr n = make_nbr<!JET>(p, x, y, z);
         static_for<7>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             int slot;
-            uint32_t i;
-            in_slot<q, PAR>(n, c, slot, i);
-            g[k][q] = Pop<HALF>::ld(p.g, p.sN, slot, i);
+            Loc l;
+            in_slot<q, PAR, true>(n, slot, l);
+            g[k][q] = ldg<HALF>(p, slot, l);
             if constexpr (HALF) g[k][q] += p.wphi0[q];
         });
     }
@@ -385,7 +439,8 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         const float stCurv = -p.sigma * curvature * ind;
         const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
 
-        const Nbr nb = make_nbr<!JET>(x, y, z, nx, ny, nz);
+        const Nbr nb = make_nbr<!JET>(p, x, y, z);
+        const Loc sf = self_f(nb), sg = self_g(nb);
 
         // ---- relaxation (src/lbm.cu:158-172)
         float omega;
@@ -411,9 +466,9 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             static_for<Q>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if constexpr (PAR == 0) {
-                    pop[q] = Pop<HALF>::ld(p.f, p.sN, q, c);
+                    pop[q] = ldf<HALF>(p, q, sf);
                 } else {
-                    pop[q] = Pop<HALF>::ld(p.f, p.sN, opp(q), nb_idx<q, -1>(nb));
+                    pop[q] = ldf<HALF>(p, opp(q), nb_f<q, -1>(nb));
                 }
             });
             rho = 0.0f;
@@ -458,24 +513,24 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 const float fneq = f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, ux, uy, uz);
                 const float fo = feq + omco * fneq + 0.5f * frc;
                 if constexpr (PAR == 0) {
-                    Pop<HALF>::st(p.f, p.sN, opp(q), c, fo);
+                    stf<HALF>(p, opp(q), sf, fo);
                 } else {
-                    Pop<HALF>::st(p.f, p.sN, q, nb_idx<q, 1>(nb), fo);
+                    stf<HALF>(p, q, nb_f<q, 1>(nb), fo);
                 }
             });
         } else {
             // ---- pair-symmetric evaluation
             // Loads: pair j holds directions a = 2j+1 (c) and b = 2j+2 (-c).
             float fa[NP], fb[NP];
-            const float f0 = Pop<HALF>::ld(p.f, p.sN, 0, c);
+            const float f0 = ldf<HALF>(p, 0, sf);
             static_for<NP>([&](auto jc) {
                 constexpr int j = decltype(jc)::value, a = 2 * j + 1, b = 2 * j + 2;
                 if constexpr (PAR == 0) {
-                    fa[j] = Pop<HALF>::ld(p.f, p.sN, a, c);
-                    fb[j] = Pop<HALF>::ld(p.f, p.sN, b, c);
+                    fa[j] = ldf<HALF>(p, a, sf);
+                    fb[j] = ldf<HALF>(p, b, sf);
                 } else {
-                    fa[j] = Pop<HALF>::ld(p.f, p.sN, b, nb_idx<a, -1>(nb));
-                    fb[j] = Pop<HALF>::ld(p.f, p.sN, a, nb_idx<a, 1>(nb));
+                    fa[j] = ldf<HALF>(p, b, nb_f<a, -1>(nb));
+                    fb[j] = ldf<HALF>(p, a, nb_f<a, 1>(nb));
                 }
             });
             // Moments from pair sums and differences: rho = 1 + sum f,
@@ -544,17 +599,17 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 k.X7s = 6.0f * (ux * pyz + uy * pxz + uz * pxy);
             }
             // rest direction: f_eq + F_0/2 (its regularized part is -cs^2 tr P = 0)
-            Pop<HALF>::st(p.f, p.sN, 0, c, wk<Q>(0) * k.B0);
+            stf<HALF>(p, 0, sf, wk<Q>(0) * k.B0);
             static_for<NP>([&](auto jc) {
                 constexpr int j = decltype(jc)::value, a = 2 * j + 1, b = 2 * j + 2;
                 float op, om;
                 pair_out<Q, kCX[a], kCY[a], kCZ[a]>(k, ux, uy, uz, fsx, fsy, fsz, op, om);
                 if constexpr (PAR == 0) {
-                    Pop<HALF>::st(p.f, p.sN, b, c, op);
-                    Pop<HALF>::st(p.f, p.sN, a, c, om);
+                    stf<HALF>(p, b, sf, op);
+                    stf<HALF>(p, a, sf, om);
                 } else {
-                    Pop<HALF>::st(p.f, p.sN, a, nb_idx<a, 1>(nb), op);
-                    Pop<HALF>::st(p.f, p.sN, b, nb_idx<a, -1>(nb), om);
+                    stf<HALF>(p, a, nb_f<a, 1>(nb), op);
+                    stf<HALF>(p, b, nb_f<a, -1>(nb), om);
                 }
             });
         }
@@ -567,9 +622,9 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 float go = g_post<q>(phi, ux, uy, uz, sharp, nrx, nry, nrz);
                 if constexpr (HALF) go -= p.wphi0[q];
                 if constexpr (PAR == 0) {
-                    Pop<HALF>::st(p.g, p.sN, opp(q), c, go);
+                    stg<HALF>(p, opp(q), sg, go);
                 } else {
-                    Pop<HALF>::st(p.g, p.sN, q, nb_idx<q, 1>(nb), go);
+                    stg<HALF>(p, q, nb_g<q, 1>(nb), go);
                 }
             });
         } else {
@@ -579,7 +634,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                                   0.125f * (p4 * uz + sharp * nrz)};
             float g0 = 0.25f * phi;
             if constexpr (HALF) g0 -= p.wphi0[0];
-            Pop<HALF>::st(p.g, p.sN, 0, c, g0);
+            stg<HALF>(p, 0, sg, g0);
             static_for<3>([&](auto ac) {
                 constexpr int ax = decltype(ac)::value, a = 2 * ax + 1, b = 2 * ax + 2;
                 float gp = ge + go_[ax], gm = ge - go_[ax];
@@ -588,11 +643,11 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                     gm -= p.wphi0[b];
                 }
                 if constexpr (PAR == 0) {
-                    Pop<HALF>::st(p.g, p.sN, b, c, gp);
-                    Pop<HALF>::st(p.g, p.sN, a, c, gm);
+                    stg<HALF>(p, b, sg, gp);
+                    stg<HALF>(p, a, sg, gm);
                 } else {
-                    Pop<HALF>::st(p.g, p.sN, a, nb_idx<a, 1>(nb), gp);
-                    Pop<HALF>::st(p.g, p.sN, b, nb_idx<a, -1>(nb), gm);
+                    stg<HALF>(p, a, nb_g<a, 1>(nb), gp);
+                    stg<HALF>(p, b, nb_g<a, -1>(nb), gm);
                 }
             });
         }
@@ -689,8 +744,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         const float tux = b[1 * plane + pi], tuy = b[2 * plane + pi], tuz = b[3 * plane + pi];
         const float pxx = b[4 * plane + pi], pyy = b[5 * plane + pi], pzz = b[6 * plane + pi];
         const float pxy = b[7 * plane + pi], pxz = b[8 * plane + pi], pyz = b[9 * plane + pi];
-        const uint32_t c = pi + plane;  // z = 1
-        const Nbr nb = make_nbr<false>(x, y, 1, nx, ny, nz);
+        const Nbr nb = make_nbr<false>(p, x, y, 1);
         static_for<Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (kCZ[q] == 1) {
@@ -704,9 +758,9 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                     val = f_eq<Q, q>(1.0f, uu, cu) + p.omco_zmin * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz);
                 }
                 int slot;
-                uint32_t i;
-                in_slot<q, NPAR>(nb, c, slot, i);
-                Pop<HALF>::st(p.f, p.sN, slot, i, val);
+                Loc l;
+                in_slot<q, NPAR, false>(nb, slot, l);
+                stf<HALF>(p, slot, l, val);
             }
         });
         float g5 = 0.0f;
@@ -723,9 +777,9 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
             }
         }
         int slot;
-        uint32_t i;
-        in_slot<5, NPAR>(nb, c, slot, i);
-        Pop<HALF>::st(p.g, p.sN, slot, i, HALF ? g5 - p.wphi0[5] : g5);
+        Loc l;
+        in_slot<5, NPAR, true>(nb, slot, l);
+        stg<HALF>(p, slot, l, HALF ? g5 - p.wphi0[5] : g5);
     }
 
     // ---------------- outflow, target cell (x, y, nz-2) ----------------
@@ -737,7 +791,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         const float tfx = b[10 * plane + pi], tfy = b[11 * plane + pi], tfz = b[12 * plane + pi];
         const uint32_t zoff = plane * (uint32_t)(nz - 2);
         const uint32_t c = pi + zoff;
-        const Nbr nb = make_nbr<false>(x, y, nz - 2, nx, ny, nz);
+        const Nbr nb = make_nbr<false>(p, x, y, nz - 2);
         const float* phi_g = F[10];
         const int xw3[3] = {(x == 1) ? nx - 2 : x - 1, x, (x == nx - 2) ? 1 : x + 1};
         const int yw3[3] = {(y == 1) ? ny - 2 : y - 1, y, (y == ny - 2) ? 1 : y + 1};
@@ -760,18 +814,18 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                                   omco_zmax * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz) +
                                   0.5f * force_of<Q, q>(cu, ux, uy, uz, tfx, tfy, tfz);
                 int slot;
-                uint32_t i;
-                in_slot<q, NPAR>(nb, c, slot, i);
-                Pop<HALF>::st(p.f, p.sN, slot, i, val);
+                Loc l;
+                in_slot<q, NPAR, false>(nb, slot, l);
+                stf<HALF>(p, slot, l, val);
             }
         });
         const float phi_b = phi_g[c];
         const float uz_b = b[3 * plane + pi];
         const float g6 = wq<7>(6) * phi_b * (1.0f - kAs2g * uz_b);
         int slot;
-        uint32_t i;
-        in_slot<6, NPAR>(nb, c, slot, i);
-        Pop<HALF>::st(p.g, p.sN, slot, i, HALF ? g6 - p.wphi0[6] : g6);
+        Loc l;
+        in_slot<6, NPAR, true>(nb, slot, l);
+        stg<HALF>(p, slot, l, HALF ? g6 - p.wphi0[6] : g6);
         // zero-gradient ghost plane z = nz-1
         const uint32_t cg = c + plane;
         F[10][cg] = phi_b;

@@ -292,9 +292,11 @@ __global__ __launch_bounds__(64) void k_mfma_layout(const uint16_t* a16, const u
 
 // ---------------------------------------------------------------------------
 // k_closure: network curvature at the interface cells of the current phi.
-// A workgroup covers 64 x 4 cells over kClZ planes; it compacts the cells
-// with mlp_lo < phi < mlp_hi, loads the weights only if it has any, and runs
-// them through the network 16 per wavefront job.
+// A workgroup covers 64 x 4 cells over kClZ planes.  It reads the phi of all
+// its planes at once and leaves if none of its cells has
+// mlp_lo < phi < mlp_hi (most workgroups); otherwise it compacts those cells
+// plane by plane, loads the weights once, and runs them through the network
+// 16 per wavefront job.
 // ---------------------------------------------------------------------------
 constexpr int kClZ = 8;
 
@@ -308,14 +310,20 @@ __global__ __launch_bounds__(256) void k_closure(KParams p) {
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const float* phi_g = p.fld[10];
     if (tid == 0) s_loaded = 0;
+    uint32_t wants = 0;  // bit k: the cell of plane k is an interface cell
+#pragma unroll
+    for (int k = 0; k < kClZ; ++k) {
+        const int z = 1 + blockIdx.z * kClZ + k;
+        if (z <= nz - 2 && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2) {
+            const float ph = phi_g[(uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z)];
+            if (ph > p.mlp_lo && ph < p.mlp_hi) wants |= 1u << k;
+        }
+    }
+    if (!__syncthreads_or(wants != 0)) return;  // uniform: no interface cell in this workgroup
     for (int k = 0; k < kClZ; ++k) {
         const int z = 1 + blockIdx.z * kClZ + k;
         if (z > nz - 2) break;  // uniform over the workgroup
-        bool want = false;
-        if (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2) {
-            const float ph = phi_g[(uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z)];
-            want = ph > p.mlp_lo && ph < p.mlp_hi;
-        }
+        const bool want = (wants >> k) & 1u;
         const unsigned long long mask = __ballot(want);
         const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
         if (lane == 0) s_count[wave] = __popcll(mask);

@@ -32,9 +32,9 @@ enum pf_velocity_set { PF_D3Q19 = 19, PF_D3Q27 = 27 };
  * src/constants.cuh:45-49, src/flowCase/{Jet,Droplet}.cuh). */
 enum pf_flow_case { PF_JET = 0, PF_DROPLET = 1 };
 
-/* Population storage.  FP32 is the plain path.  FP16 stores the shifted
- * populations f - w*rho0 and g - w*phi0 in binary16 (round to nearest even)
- * with FP32 arithmetic, the role of ENABLE_FP16 (src/cuda/precision.cuh:29-42),
+/* Population storage.  FP32 is the plain path.  FP16 stores f - w*rho0 and
+ * g_q - w_q*phi(destination cell) in binary16 (round to nearest even) with
+ * FP32 arithmetic, the role of ENABLE_FP16 (src/cuda/precision.cuh:29-42),
  * which in the reference stores only f in FP16. */
 enum pf_storage { PF_STORE_FP32 = 0, PF_STORE_FP16 = 1 };
 
@@ -107,7 +107,7 @@ typedef struct pf_config {
     /* phase field */
     float interface_width;
     float tau_g;           /* only sets the sharpening gamma, as in the reference */
-    float phi0;            /* FP16 storage shift of g (g - w*phi0); default 0 */
+    float phi0;            /* unused: FP16 storage keeps g_q relative to w_q*phi of the cell it streams to */
     /* execution */
     int32_t device;        /* HIP device index; <0 reads GPU_INDEX like setDeviceFromEnv */
     int32_t zchunk;        /* z cells marched per workgroup; 0 = automatic */

@@ -237,7 +237,6 @@ struct Consts {
     T sp_z_start, sp_inv_nz_m1, sp_inv_sponge;
     // geometry (globalFunctions.cuh:104-125)
     T cx, cy, cz, R2;
-    T wphi0[7];
 };
 
 template <class T>
@@ -311,7 +310,6 @@ struct Sim final : Base {
         k.cx = (T)(nx - 1) * (T)0.5;
         k.cy = (T)(ny - 1) * (T)0.5;
         k.cz = (T)(nz - 1) * (T)0.5;
-        for (int q = 0; q < 7; ++q) k.wphi0[q] = HALF ? (T)((float)G.w[q] * c.phi0) : (T)0;
     }
 
     size_t id3(int x, int y, int z) const { return (size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z); }
@@ -328,12 +326,16 @@ struct Sim final : Base {
         if constexpr (HALF) f[(size_t)q * N + i] = f2h((float)v);
         else f[(size_t)q * N + i] = v;
     }
+    // binary16 storage keeps g_q of cell i relative to w_q * phi(i), the phi
+    // field of the step that stored it (the destination cell's phi): the
+    // stored numbers are the small velocity and gradient parts of g, so
+    // their rounding carries no systematic bias from the bulk value w_q*phi.
     T ld_g(int q, size_t i) const {
-        if constexpr (HALF) return (T)(h2f(g[(size_t)q * N + i]) + (float)k.wphi0[q]);
+        if constexpr (HALF) return (T)(h2f(g[(size_t)q * N + i]) + (float)G.w[q] * (float)fld[PF_FIELD_PHI][i]);
         else return g[(size_t)q * N + i];
     }
     void st_g(int q, size_t i, T v) {
-        if constexpr (HALF) g[(size_t)q * N + i] = f2h((float)(v - k.wphi0[q]));
+        if constexpr (HALF) g[(size_t)q * N + i] = f2h((float)v - (float)G.w[q] * (float)fld[PF_FIELD_PHI][i]);
         else g[(size_t)q * N + i] = v;
     }
 
@@ -586,7 +588,14 @@ struct Sim final : Base {
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 const T dx = (T)x - k.cx, dy = (T)y - k.cy;
-                if (dx * dx + dy * dy > k.R2) continue;
+                if (dx * dx + dy * dy > k.R2) {
+                    // Frozen wall: no population reaches z = 1 from z = 0 off the
+                    // nozzle, so g_5 there keeps its initial value 0.  binary16
+                    // storage is relative to the current phi of the cell, so the
+                    // zero is stored again each step.
+                    if (HALF && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2) st_g(5, id3(x, y, 1), (T)0);
+                    continue;
+                }
                 const size_t ib = id3(x, y, 0);
                 T ux = 0, uy = 0, uz = k.u_inf;
                 if (c.noise == PF_NOISE_GAUSSIAN_XY) {

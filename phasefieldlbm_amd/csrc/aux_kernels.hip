@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
     static_for<7>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         float v = g_post<q>(phi, ux, uy, uz, 0.0f, 0.0f, 0.0f, 0.0f);
-        if constexpr (HALF) v -= p.wphi0[q];
+        if constexpr (HALF) v -= wq<7>(q) * phi;  // binary16: relative to w_q * phi of the cell
         st_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, p.nx, q, x, y, z), v);
     });
 }
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
                 v = ld_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, nx, slot, sx, sy, sz));
             } else {
                 v = ld_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, nx, slot, sx, sy, sz));
-                if constexpr (HALF) v += p.wphi0[q];
+                if constexpr (HALF) v += wq<7>(q) * p.fld[10][i];
             }
         }
         out[(size_t)q * p.N + i] = v;
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials)
                 sz = zs[1 - kCZ[q]];
             }
             float gv = ld_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, nx, slot, sx, sy, sz));
-            if constexpr (HALF) gv += p.wphi0[q];
+            if constexpr (HALF) gv += wq<7>(q) * p.fld[10][i];
             phi += (double)gv;
         }
         v[0] += phi;

@@ -71,7 +71,6 @@ struct KParams {
     float omega_ref;
     float sp_z_start, sp_inv_nz_m1, sp_inv_sponge;
     float cxg, cyg, R2;  // nozzle centre and radius^2
-    float wphi0[7];      // FP16 storage shift of g
     // inflow noise
     int noise;
     float noise_amp;

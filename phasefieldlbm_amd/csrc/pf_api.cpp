@@ -136,8 +136,6 @@ void fill_constants(pf_ctx* c) {
     }
     k.cxg = (float)(g.nx - 1) * 0.5f;
     k.cyg = (float)(g.ny - 1) * 0.5f;
-    const float w7[7] = {0.25f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f, 0.125f};
-    for (int q = 0; q < 7; ++q) k.wphi0[q] = c->half ? w7[q] * g.phi0 : 0.0f;
     k.noise = g.noise;
     k.noise_amp = g.noise_amp;
     k.seed = g.seed;

@@ -217,8 +217,18 @@ __global__ __launch_bounds__(kThreads) void k_phase_sum(KParams p) {
             Loc l;
             in_slot<q, PAR, true>(n, slot, l);
             g[k][q] = ldg<HALF>(p, slot, l);
-            if constexpr (HALF) g[k][q] += p.wphi0[q];
         });
+    }
+    if constexpr (HALF) {
+        // binary16: stored g is relative to w_q * (this cell's phi of the
+        // previous step), which the phi field still holds
+#pragma unroll
+        for (int k = 0; k < kPsZ; ++k) {
+            const int z = min(z0 + k, p.nz - 2);
+            const float po = p.fld[10][(uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z)];
+#pragma unroll
+            for (int q = 0; q < 7; ++q) g[k][q] += wq<7>(q) * po;
+        }
     }
 #pragma unroll
     for (int k = 0; k < kPsZ; ++k) {
@@ -427,6 +437,32 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         }
         if (!active) continue;
         const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+        // binary16 storage keeps g_q relative to w_q * phi of the cell it
+        // streams to (the reference's periodicity): phi of the six face
+        // neighbours, from the LDS tile unless the neighbour wraps; read here,
+        // with the other LDS reads, and used by the g stores.
+        float phn[6] = {0, 0, 0, 0, 0, 0};  // -x, +x, -y, +y, -z, +z
+        if constexpr (HALF) {
+            const float* pl = s_phi[z % 5];
+            const int o2 = (ty + 2) * kHX + (tx + 2);
+            phn[0] = pl[o2 - 1];
+            phn[1] = pl[o2 + 1];
+            phn[2] = pl[o2 - kHX];
+            phn[3] = pl[o2 + kHX];
+            phn[4] = s_phi[(z + 4) % 5][o2];
+            phn[5] = s_phi[(z + 1) % 5][o2];
+            auto at = [&](int gx, int gy, int gz) {
+                return phi_g[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)gz)];
+            };
+            if (x == 1) phn[0] = at(nx - 2, y, z);
+            if (x == nx - 2) phn[1] = at(1, y, z);
+            if (y == 1) phn[2] = at(x, ny - 2, z);
+            if (y == ny - 2) phn[3] = at(x, 1, z);
+            if constexpr (!JET) {
+                if (z == 1) phn[4] = at(x, y, nz - 2);
+                if (z == nz - 2) phn[5] = at(x, y, 1);
+            }
+        }
         if constexpr (MLP) {
             // learned closure at interface cells whose FD curvature lies in
             // the trained range (evaluated by k_closure for this step)
@@ -616,11 +652,16 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
 
         // ---- phase populations (src/lbm.cu:209-238): g_eq + sharpening flux
         const float sharp = p.gamma * phi * (1.0f - phi);
+
         if constexpr (LIT) {
             static_for<7>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 float go = g_post<q>(phi, ux, uy, uz, sharp, nrx, nry, nrz);
-                if constexpr (HALF) go -= p.wphi0[q];
+                if constexpr (HALF) {
+                    constexpr int d = kCX[q] != 0 ? (kCX[q] > 0 ? 1 : 0)
+                                                  : (kCY[q] != 0 ? (kCY[q] > 0 ? 3 : 2) : (kCZ[q] > 0 ? 5 : 4));
+                    go -= wq<7>(q) * (q == 0 ? phi : phn[d]);
+                }
                 if constexpr (PAR == 0) {
                     stg<HALF>(p, opp(q), sg, go);
                 } else {
@@ -633,14 +674,14 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             const float go_[3] = {0.125f * (p4 * ux + sharp * nrx), 0.125f * (p4 * uy + sharp * nry),
                                   0.125f * (p4 * uz + sharp * nrz)};
             float g0 = 0.25f * phi;
-            if constexpr (HALF) g0 -= p.wphi0[0];
+            if constexpr (HALF) g0 -= 0.25f * phi;  // rest population: destination is the cell itself
             stg<HALF>(p, 0, sg, g0);
             static_for<3>([&](auto ac) {
                 constexpr int ax = decltype(ac)::value, a = 2 * ax + 1, b = 2 * ax + 2;
                 float gp = ge + go_[ax], gm = ge - go_[ax];
-                if constexpr (HALF) {
-                    gp -= p.wphi0[a];
-                    gm -= p.wphi0[b];
+                if constexpr (HALF) {  // direction a is +axis, b is -axis
+                    gp -= 0.125f * phn[2 * ax + 1];
+                    gm -= 0.125f * phn[2 * ax];
                 }
                 if constexpr (PAR == 0) {
                     stg<HALF>(p, b, sg, gp);
@@ -779,7 +820,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         int slot;
         Loc l;
         in_slot<5, NPAR, true>(nb, slot, l);
-        stg<HALF>(p, slot, l, HALF ? g5 - p.wphi0[5] : g5);
+        stg<HALF>(p, slot, l, HALF ? g5 - wq<7>(5) * F[10][pi + plane] : g5);
     }
 
     // ---------------- outflow, target cell (x, y, nz-2) ----------------
@@ -825,7 +866,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         int slot;
         Loc l;
         in_slot<6, NPAR, true>(nb, slot, l);
-        stg<HALF>(p, slot, l, HALF ? g6 - p.wphi0[6] : g6);
+        stg<HALF>(p, slot, l, HALF ? g6 - wq<7>(6) * phi_b : g6);
         // zero-gradient ghost plane z = nz-1
         const uint32_t cg = c + plane;
         F[10][cg] = phi_b;

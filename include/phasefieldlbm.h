@@ -55,14 +55,37 @@ enum pf_closure { PF_CLOSURE_FD = 0, PF_CLOSURE_MLP = 1 };
  * floating-point rounding only. */
 enum pf_kernel { PF_KERNEL_PAIR = 0, PF_KERNEL_LITERAL = 1 };
 
-/* Field identifiers; 0..17 follow host::FieldID (src/fileIO/fields.cuh:33-68). */
+/* Field identifiers in host::FieldID order (src/fileIO/fields.cuh:33-68):
+ * 0..17 the timestep's fields, 18..31 the derived fields of
+ * src/derivedFields/ (available when enabled in pf_config.derived).  The
+ * reference's passive scalar "c" is a stub that traps
+ * (src/derivedFields/operators/passiveScalar.cuh:34-38) and is not offered. */
 enum pf_field {
     PF_FIELD_RHO = 0, PF_FIELD_UX, PF_FIELD_UY, PF_FIELD_UZ,
     PF_FIELD_PXX, PF_FIELD_PYY, PF_FIELD_PZZ, PF_FIELD_PXY, PF_FIELD_PXZ, PF_FIELD_PYZ,
     PF_FIELD_PHI, PF_FIELD_NORMX, PF_FIELD_NORMY, PF_FIELD_NORMZ, PF_FIELD_IND,
     PF_FIELD_FSX, PF_FIELD_FSY, PF_FIELD_FSZ,
+    PF_NUM_BASE_FIELDS,
+    PF_FIELD_AVG_PHI = PF_NUM_BASE_FIELDS, PF_FIELD_AVG_UX, PF_FIELD_AVG_UY, PF_FIELD_AVG_UZ,
+    PF_FIELD_AVG_UXUX, PF_FIELD_AVG_UYUY, PF_FIELD_AVG_UZUZ, PF_FIELD_AVG_UXUY, PF_FIELD_AVG_UXUZ,
+    PF_FIELD_AVG_UYUZ,
+    PF_FIELD_VORT_X, PF_FIELD_VORT_Y, PF_FIELD_VORT_Z, PF_FIELD_VORT_MAG,
     PF_NUM_FIELDS
 };
+
+/* Derived fields (bit mask, pf_config.derived), the reference's
+ * TIME_AVERAGE, REYNOLDS_MOMENTS and VORTICITY_FIELDS builds
+ * (src/derivedFields/operators/timeAverage.cuh, reynoldsMoments.cuh,
+ * vorticityFields.cuh):
+ *   AVERAGE    running means of phi, ux, uy, uz over every step, updated
+ *              a += (v - a) / (t + 1) after step t, ghost cells included
+ *   REYNOLDS   running means of ux*ux, uy*uy, uz*uz, ux*uy, ux*uz, uy*uz
+ *   VORTICITY  curl u by central differences at interior cells (0 on the
+ *              ghost shell), evaluated from the last saved velocity when
+ *              read or written
+ * With AVERAGE or REYNOLDS every step materialises the fields (as if run
+ * with PF_STEP_SAVE_LAST), as the reference's per-step update needs. */
+enum pf_derived { PF_DERIVED_AVERAGE = 1, PF_DERIVED_REYNOLDS = 2, PF_DERIVED_VORTICITY = 4 };
 
 enum pf_status {
     PF_OK = 0,
@@ -115,6 +138,7 @@ typedef struct pf_config {
     int32_t kernel;        /* pf_kernel */
     int32_t slot_pad;      /* elements of padding per population row (beyond Q*nx); <0 = automatic (0) */
     float mlp_kmin, mlp_kmax; /* MLP applies where the FD curvature lies in [kmin, kmax] */
+    int32_t derived;       /* pf_derived bit mask; 0 = none (the reference's default build) */
 } pf_config;
 
 typedef struct pf_ctx pf_ctx;
@@ -133,7 +157,7 @@ void pf_destroy(pf_ctx *ctx);
  * (src/main.cu:114-116, src/initialConditions.cu:33-138). */
 int pf_init(pf_ctx *ctx);
 
-/* Replace field `id` (PF_FIELD_RHO..UZ or PHI) with host data of n =
+/* Replace field `id` (one of the first PF_NUM_BASE_FIELDS) with host data of n =
  * nx*ny*nz floats; follow with pf_init_distros.  Used to start from an
  * arbitrary state (host-computed ICs). */
 int pf_set_field(pf_ctx *ctx, int32_t id, const float *host, size_t n);
@@ -151,7 +175,8 @@ int pf_sync(pf_ctx *ctx);
 /* Copy field `id` (n = nx*ny*nz floats) to host memory.  Fields other than
  * PHI hold the values of the last step run with PF_STEP_SAVE_LAST (or the
  * initial conditions), as the reference's arrays do after that step
- * (PostProcess::bin, src/postProcess/PostProcess.cuh:43-97). */
+ * (PostProcess::bin, src/postProcess/PostProcess.cuh:43-97).  Derived fields
+ * must be enabled in pf_config.derived (else PF_ERR_CONFIG). */
 int pf_get_field(pf_ctx *ctx, int32_t id, float *host, size_t n);
 
 /* Copy the post-streaming populations f (which=0, Q*N) or g (which=1, 7*N)

@@ -23,10 +23,14 @@ STEP_SAVE_LAST = 1
 OUT_BIN, OUT_VTI, OUT_VTR = 1, 2, 4
 MLP_PARAMS = 6017
 
-# host::FieldID order (src/fileIO/fields.cuh:33-68)
+# host::FieldID order (src/fileIO/fields.cuh:33-68): the timestep's fields,
+# then the derived fields (enabled through Config.derived)
 FIELDS = ("rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz",
           "phi", "normx", "normy", "normz", "ind", "fsx", "fsy", "fsz")
-FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+DERIVED_FIELDS = ("avg_phi", "avg_ux", "avg_uy", "avg_uz", "avg_uxux", "avg_uyuy", "avg_uzuz", "avg_uxuy",
+                  "avg_uxuz", "avg_uyuz", "vort_x", "vort_y", "vort_z", "vort_mag")
+FIELD_ID = {n: i for i, n in enumerate(FIELDS + DERIVED_FIELDS)}
+DERIVED_AVERAGE, DERIVED_REYNOLDS, DERIVED_VORTICITY = 1, 2, 4
 
 STATUS = {0: "ok", -1: "invalid configuration or argument", -2: "device unavailable",
           -3: "device allocation failed", -4: "kernel launch or runtime error",
@@ -62,6 +66,7 @@ class PfConfig(ctypes.Structure):
         ("kernel", ctypes.c_int32),
         ("slot_pad", ctypes.c_int32),
         ("mlp_kmin", ctypes.c_float), ("mlp_kmax", ctypes.c_float),
+        ("derived", ctypes.c_int32),
     ]
 
 

@@ -279,6 +279,65 @@ hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parit
     return Q == 27 ? gather_q<27>(p, jet, half, parity, which, out, s) : gather_q<19>(p, jet, half, parity, which, out, s);
 }
 
+// Running means of the derived-field set over every cell, ghosts included
+// (timeAverage and reynoldsMomentsAverage, src/derivedFields/operators/
+// timeAverage.cuh:34-88, reynoldsMoments.cuh:34-104): a += (v - a) / t.
+// Each thread handles 4 consecutive cells.
+__global__ __launch_bounds__(256) void k_average(KParams p, DerivedPtrs d, float t) {
+    const uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 4u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t i = i0 + k;
+        if (i >= p.N) return;
+        const float phi = p.fld[10][i], ux = p.fld[1][i], uy = p.fld[2][i], uz = p.fld[3][i];
+        const float v[10] = {phi, ux, uy, uz, ux * ux, uy * uy, uz * uz, ux * uy, ux * uz, uy * uz};
+#pragma unroll
+        for (int a = 0; a < 10; ++a) {
+            float* f = d.avg[a];
+            if (f) f[i] = f[i] + (v[a] - f[i]) / t;
+        }
+    }
+}
+
+// Vorticity by central differences at interior cells, 0 on the ghost shell
+// (vorticityCompute, src/derivedFields/operators/vorticityFields.cuh:34-100).
+__global__ __launch_bounds__(256) void k_vorticity(KParams p, DerivedPtrs d) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.N) return;
+    const uint32_t nx = (uint32_t)p.nx, ny = (uint32_t)p.ny, nz = (uint32_t)p.nz;
+    const uint32_t x = i % nx, y = (i / nx) % ny, z = i / (nx * ny);
+    float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+    if (x > 0 && x < nx - 1 && y > 0 && y < ny - 1 && z > 0 && z < nz - 1) {
+        const float* ux = p.fld[1];
+        const float* uy = p.fld[2];
+        const float* uz = p.fld[3];
+        const uint32_t sy = nx, sz = nx * ny;
+        const float duz_dy = 0.5f * (uz[i + sy] - uz[i - sy]);
+        const float duy_dz = 0.5f * (uy[i + sz] - uy[i - sz]);
+        const float dux_dz = 0.5f * (ux[i + sz] - ux[i - sz]);
+        const float duz_dx = 0.5f * (uz[i + 1] - uz[i - 1]);
+        const float duy_dx = 0.5f * (uy[i + 1] - uy[i - 1]);
+        const float dux_dy = 0.5f * (ux[i + sy] - ux[i - sy]);
+        wx = duz_dy - duy_dz;
+        wy = dux_dz - duz_dx;
+        wz = duy_dx - dux_dy;
+    }
+    if (d.vort[0]) d.vort[0][i] = wx;
+    if (d.vort[1]) d.vort[1][i] = wy;
+    if (d.vort[2]) d.vort[2][i] = wz;
+    if (d.vort[3]) d.vort[3][i] = sqrtf(wx * wx + wy * wy + wz * wz);
+}
+
+hipError_t launch_average(const KParams& p, const DerivedPtrs& d, float t, hipStream_t s) {
+    hipLaunchKernelGGL(k_average, dim3((p.N + 1023u) / 1024u), dim3(256), 0, s, p, d, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_vorticity(const KParams& p, const DerivedPtrs& d, hipStream_t s) {
+    hipLaunchKernelGGL(k_vorticity, dim3((p.N + 255u) / 256u), dim3(256), 0, s, p, d);
+    return hipGetLastError();
+}
+
 hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_diagnostics, dim3(nblocks), dim3(256), 0, s, p, partials);
     return hipGetLastError();

@@ -100,6 +100,16 @@ hipError_t launch_phase_mass(const KParams& p, bool jet, bool half, int parity, 
                              hipStream_t s);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
+
+// Derived fields (src/derivedFields/): avg[0..3] running means of phi, ux,
+// uy, uz; avg[4..9] of ux*ux, uy*uy, uz*uz, ux*uy, ux*uz, uy*uz; vort[0..3]
+// curl u and its magnitude.  Null pointers are skipped.
+struct DerivedPtrs {
+    float* avg[10];
+    float* vort[4];
+};
+hipError_t launch_average(const KParams& p, const DerivedPtrs& d, float t, hipStream_t s);
+hipError_t launch_vorticity(const KParams& p, const DerivedPtrs& d, hipStream_t s);
 hipError_t launch_mlp_grad(const uint16_t* w16, const float* bias, const float* xs, const float* ts, const int* idx,
                            const uint8_t* symperm, int nsym, uint32_t sym_seed, size_t n, size_t s_begin,
                            size_t count, float scale, float* grad_out, hipStream_t s);

@@ -37,6 +37,7 @@ struct pf_ctx {
     int nred = 0;
     float* gather_buf = nullptr;
     float* kappa = nullptr;
+    float* der[PF_NUM_FIELDS - PF_NUM_BASE_FIELDS] = {};  // derived fields, by id - PF_NUM_BASE_FIELDS
     uint16_t* mlp_w = nullptr;
     float* mlp_b = nullptr;
     bool has_mlp = false;
@@ -238,6 +239,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     if (!(g.interface_width > 0) || g.axis_x == 0 || g.axis_y == 0 || g.axis_z == 0) return PF_ERR_CONFIG;
     if (g.noise < 0 || g.noise > 2) return PF_ERR_CONFIG;
     if (g.kernel != PF_KERNEL_PAIR && g.kernel != PF_KERNEL_LITERAL) return PF_ERR_CONFIG;
+    if (g.derived & ~(PF_DERIVED_AVERAGE | PF_DERIVED_REYNOLDS | PF_DERIVED_VORTICITY)) return PF_ERR_CONFIG;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
     if (N >= (size_t)1 << 30) return PF_ERR_CONFIG;  // byte offsets of one slot stay below 2^32
 
@@ -302,6 +304,13 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     ok = ok && alloc((void**)&c->partials, (size_t)c->nred * 6 * sizeof(double));
     ok = ok && alloc((void**)&c->dout, 8 * sizeof(double));
     if (g.closure == PF_CLOSURE_MLP) ok = ok && alloc((void**)&c->kappa, N * sizeof(float));
+    for (int i = 0; ok && i < PF_NUM_FIELDS - PF_NUM_BASE_FIELDS; ++i) {
+        const int id = PF_NUM_BASE_FIELDS + i;
+        const bool want = (id <= PF_FIELD_AVG_UZ && (g.derived & PF_DERIVED_AVERAGE)) ||
+                          (id >= PF_FIELD_AVG_UXUX && id <= PF_FIELD_AVG_UYUZ && (g.derived & PF_DERIVED_REYNOLDS)) ||
+                          (id >= PF_FIELD_VORT_X && (g.derived & PF_DERIVED_VORTICITY));
+        if (want) ok = alloc((void**)&c->der[i], N * sizeof(float));
+    }
     if (!ok) {
         pf_destroy(c);
         return PF_ERR_OOM;
@@ -344,6 +353,7 @@ void pf_destroy(pf_ctx* c) {
     (void)hipFree(c->dout);
     (void)hipFree(c->gather_buf);
     (void)hipFree(c->kappa);
+    for (auto* p : c->der) (void)hipFree(p);
     (void)hipFree(c->mlp_w);
     (void)hipFree(c->mlp_b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -369,6 +379,8 @@ int pf_init(pf_ctx* c) {
     const pf_config& g = c->cfg;
     for (int i = 0; i < pf::kNumFields; ++i)
         PF_HIP(c, hipMemsetAsync(c->fld[i], 0, c->N * sizeof(float), c->stream));
+    for (auto* d : c->der)
+        if (d) PF_HIP(c, hipMemsetAsync(d, 0, c->N * sizeof(float), c->stream));
     PF_HIP(c, hipMemsetAsync(c->f, 0, c->f_elems * pop_bytes(c), c->stream));
     PF_HIP(c, hipMemsetAsync(c->g, 0, c->g_elems * pop_bytes(c), c->stream));
     PF_HIP(c, pf::launch_init_fields(c->kp, c->jet, g.radius, g.axis_x, g.axis_y, g.axis_z, g.interface_width,
@@ -402,7 +414,7 @@ int pf_init(pf_ctx* c) {
 
 int pf_set_field(pf_ctx* c, int32_t id, const float* host, size_t n) {
     if (!c || !host) return PF_ERR_CONFIG;
-    if (id < 0 || id >= PF_NUM_FIELDS) return fail(c, PF_ERR_CONFIG, "bad field id");
+    if (id < 0 || id >= PF_NUM_BASE_FIELDS) return fail(c, PF_ERR_CONFIG, "bad field id");
     if (n != c->N) return fail(c, PF_ERR_CONFIG, "size mismatch");
     PF_HIP(c, hipSetDevice(c->dev));
     PF_HIP(c, hipMemcpyAsync(c->fld[id], host, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -410,10 +422,21 @@ int pf_set_field(pf_ctx* c, int32_t id, const float* host, size_t n) {
     return PF_OK;
 }
 
+static pf::DerivedPtrs derived_ptrs(const pf_ctx* c) {
+    pf::DerivedPtrs d{};
+    for (int i = 0; i < 10; ++i) d.avg[i] = c->der[i];
+    for (int i = 0; i < 4; ++i) d.vort[i] = c->der[10 + i];
+    return d;
+}
+
 static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hipEvent_t* ev) {
+    // Running means need the macroscopic fields of every step (the reference
+    // computes them every step and averages after the boundary conditions).
+    const bool avg = (c->cfg.derived & (PF_DERIVED_AVERAGE | PF_DERIVED_REYNOLDS)) != 0;
+    const pf::DerivedPtrs dp = derived_ptrs(c);
     for (uint32_t s = 0; s < nsteps; ++s) {
         c->kp.t = (uint32_t)(t0 + s);
-        c->kp.save = ((flags & PF_STEP_SAVE_LAST) && s + 1 == nsteps) ? 1 : 0;
+        c->kp.save = (avg || ((flags & PF_STEP_SAVE_LAST) && s + 1 == nsteps)) ? 1 : 0;
         const int variant = c->cfg.closure == PF_CLOSURE_MLP ? 2 : c->cfg.kernel;
         if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
         PF_HIP(c, pf::launch_phase_sum(c->kp, c->Q, c->jet, c->half, c->parity, c->stream));
@@ -422,6 +445,7 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
         PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[2], c->stream));
         if (c->jet) PF_HIP(c, pf::launch_jet_bc(c->kp, c->Q, c->half, c->parity ^ 1, c->stream));
+        if (avg) PF_HIP(c, pf::launch_average(c->kp, dp, (float)(t0 + s + 1), c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[3], c->stream));
         c->parity ^= 1;
         if (ev) ev += 4;
@@ -449,7 +473,15 @@ int pf_get_field(pf_ctx* c, int32_t id, float* host, size_t n) {
     if (id < 0 || id >= PF_NUM_FIELDS) return fail(c, PF_ERR_CONFIG, "bad field id");
     if (n != c->N) return fail(c, PF_ERR_CONFIG, "size mismatch");
     PF_HIP(c, hipSetDevice(c->dev));
-    PF_HIP(c, hipMemcpyAsync(host, c->fld[id], n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    const float* src = c->fld[0];
+    if (id < PF_NUM_BASE_FIELDS) {
+        src = c->fld[id];
+    } else {
+        src = c->der[id - PF_NUM_BASE_FIELDS];
+        if (!src) return fail(c, PF_ERR_CONFIG, "derived field not enabled in pf_config.derived");
+        if (id >= PF_FIELD_VORT_X) PF_HIP(c, pf::launch_vorticity(c->kp, derived_ptrs(c), c->stream));
+    }
+    PF_HIP(c, hipMemcpyAsync(host, src, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     PF_HIP(c, hipStreamSynchronize(c->stream));
     return PF_OK;
 }
@@ -527,8 +559,10 @@ int pf_set_closure(pf_ctx* c, const float* W1, const float* b1, const float* W2,
 }
 
 namespace {
-const char* const kFieldNames[PF_NUM_FIELDS] = {"rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz",
-                                                 "pyz", "phi", "normx", "normy", "normz", "ind", "fsx", "fsy", "fsz"};
+const char* const kFieldNames[PF_NUM_FIELDS] = {
+    "rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz", "phi", "normx", "normy", "normz", "ind",
+    "fsx", "fsy", "fsz", "avg_phi", "avg_ux", "avg_uy", "avg_uz", "avg_uxux", "avg_uyuy", "avg_uzuz", "avg_uxuy",
+    "avg_uxuz", "avg_uyuz", "vort_x", "vort_y", "vort_z", "vort_mag"};
 }
 
 const char* pf_field_name(int32_t id) { return id >= 0 && id < PF_NUM_FIELDS ? kFieldNames[id] : nullptr; }

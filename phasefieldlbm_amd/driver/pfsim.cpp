@@ -47,6 +47,8 @@ void usage(const char* argv0) {
               << "  --fp16         binary16 population storage\n"
               << "  --noise gaussian|uniform|none, --noise-amp A, --seed N\n"
               << "  --closure FILE learned curvature closure weights (6017 little-endian float32)\n"
+              << "  --derived LIST average,reynolds,vorticity: derived fields, appended to the output\n"
+              << "                 (the reference's TIME_AVERAGE, REYNOLDS_MOMENTS, VORTICITY_FIELDS)\n"
               << "  --fields LIST  output fields, comma separated (default rho,uz)\n"
               << "  --format LIST  bin,vti,vtr (default bin,vtr)\n"
               << "  --root DIR     directory holding bin/ (default: current directory)\n";
@@ -234,6 +236,15 @@ int main(int argc, char* argv[]) {
         else if (a == "--seed") { need(1); cfg.seed = (uint32_t)std::strtoul(argv[++i], nullptr, 0); }
         else if (a == "--closure") { need(1); r.closure_file = argv[++i]; }
         else if (a == "--fields") { need(1); fields = argv[++i]; }
+        else if (a == "--derived") {
+            need(1);
+            for (const std::string& d : split(argv[++i])) {
+                if (d == "average") cfg.derived |= PF_DERIVED_AVERAGE;
+                else if (d == "reynolds") cfg.derived |= PF_DERIVED_REYNOLDS;
+                else if (d == "vorticity") cfg.derived |= PF_DERIVED_VORTICITY;
+                else { std::cerr << "Error: unknown derived field set '" << d << "'\n"; return 1; }
+            }
+        }
         else if (a == "--format") { need(1); formats = argv[++i]; }
         else if (a == "--root") { need(1); r.root = argv[++i]; }
         else { std::cerr << "Error: unknown option '" << a << "'\n"; usage(argv[0]); return 1; }
@@ -253,6 +264,15 @@ int main(int argc, char* argv[]) {
         const int id = field_id(f);
         if (id < 0) { std::cerr << "Error: unknown field '" << f << "'\n"; return 1; }
         r.fields.push_back(id);
+    }
+    // enabled derived fields follow the base fields (src/main.cu:130-146)
+    for (int32_t id = PF_NUM_BASE_FIELDS; id < PF_NUM_FIELDS; ++id) {
+        const bool on = (id <= PF_FIELD_AVG_UZ && (cfg.derived & PF_DERIVED_AVERAGE)) ||
+                        (id >= PF_FIELD_AVG_UXUX && id <= PF_FIELD_AVG_UYUZ && (cfg.derived & PF_DERIVED_REYNOLDS)) ||
+                        (id >= PF_FIELD_VORT_X && (cfg.derived & PF_DERIVED_VORTICITY));
+        bool listed = false;
+        for (int32_t f : r.fields) listed = listed || f == id;
+        if (on && !listed) r.fields.push_back(id);
     }
     r.formats = 0;
     for (const std::string& f : split(formats)) {

@@ -91,9 +91,13 @@ def test_vtk_rejects_bad_arguments(tmp_path):
 
 
 def test_field_names():
+    """Names in host::FieldID order (src/fileIO/fields.cuh:33-68), derived
+    fields after the timestep's; the reference's passive scalar is not offered."""
     lib = _abi.load()
-    assert [lib.pf_field_name(i).decode() for i in range(len(_abi.FIELDS))] == list(_abi.FIELDS)
-    assert lib.pf_field_name(-1) is None and lib.pf_field_name(len(_abi.FIELDS)) is None
+    names = list(_abi.FIELDS + _abi.DERIVED_FIELDS)
+    assert [lib.pf_field_name(i).decode() for i in range(len(names))] == names
+    assert names[18] == "avg_phi" and names[28] == "vort_x" and names[31] == "vort_mag"
+    assert lib.pf_field_name(-1) is None and lib.pf_field_name(len(names)) is None
 
 
 def _run(args, cwd, timeout=300):
@@ -178,3 +182,18 @@ def test_driver_learned_closure(tmp_path):
     d = tmp_path / "bin" / "DROPLET" / "D3Q19" / "002"
     rho = vtk.read_bin(str(d / "rho000030.bin"), (64, 64, 64))
     assert np.all(np.isfinite(rho))
+
+
+@pytest.mark.gpu
+def test_driver_derived_fields(tmp_path):
+    """--derived appends the enabled derived fields to the output, as the
+    reference's main appends DerivedFields::makeOutputFields (src/main.cu:130-146)."""
+    r = _run(["DROPLET", "D3Q19", "003", "--preset", "droplet_d3q19_64", "--steps", "4", "--save", "4",
+              "--derived", "average,vorticity", "--format", "bin"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = tmp_path / "bin" / "DROPLET" / "D3Q19" / "003"
+    for k in ("rho", "uz", "avg_phi", "avg_ux", "avg_uy", "avg_uz", "vort_x", "vort_y", "vort_z", "vort_mag"):
+        assert (d / f"{k}000004.bin").exists(), k
+    assert not (d / "avg_uxux000004.bin").exists()
+    r = _run(["DROPLET", "D3Q19", "004", "--derived", "scalar"], tmp_path)
+    assert r.returncode == 1 and "derived" in r.stderr

@@ -249,6 +249,16 @@ def load(path=WEIGHTS):
         return z["params"].astype(np.float32)
 
 
+def predict_curvature(params, X, device=0):
+    """Curvature the closure applies for stencils X [n, 27]: the network
+    averaged over each stencil and its point reflection (feature
+    f <-> 26 - f), as the timestep's closure pass evaluates it."""
+    net = MLP(device, params)
+    y = 0.5 * (net.predict(X).astype(np.float64) + net.predict(np.ascontiguousarray(X[:, ::-1])).astype(np.float64))
+    net.close()
+    return 0.25 * y
+
+
 def curvature_error(params, radii=(6, 8, 12, 16, 20), steps=500, n=64, velocity_set=D3Q19, sigma=1e-3):
     """Pointwise curvature of relaxed static spheres against the exact -2/R:
     the finite-difference div(n) the timestep computes and the network on the
@@ -268,9 +278,7 @@ def curvature_error(params, radii=(6, 8, 12, 16, 20), steps=500, n=64, velocity_
         cells = np.flatnonzero(sel.ravel())
         n2 = (nv ** 2).sum(0).ravel()[cells]
         k_fd = -(fs * nv).sum(0).ravel()[cells] / (sigma * ind.ravel()[cells] * n2)
-        net = MLP(0, params)
-        k_mlp = 0.25 * net.predict(stencils(phi, cells)).astype(np.float64)
-        net.close()
+        k_mlp = predict_curvature(params, stencils(phi, cells))
         want = -2.0 / R
         row = {"R": R, "cells": int(cells.size)}
         for name, k in (("fd", k_fd), ("mlp", k_mlp)):

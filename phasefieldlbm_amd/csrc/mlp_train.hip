@@ -294,14 +294,17 @@ __global__ __launch_bounds__(64) void k_mfma_layout(const uint16_t* a16, const u
 // k_closure: network curvature at the interface cells of the current phi.
 // A workgroup covers 64 x 4 cells over kClZ planes.  It reads the phi of all
 // its planes at once and leaves if none of its cells has
-// mlp_lo < phi < mlp_hi (most workgroups); otherwise it compacts those cells
-// plane by plane, loads the weights once, and runs them through the network
-// 16 per wavefront job.
+// mlp_lo < phi < mlp_hi (most workgroups).  Otherwise it marches its planes
+// with phi of planes z-1..z+1 (1-cell halo) staged in LDS, compacts the
+// interface cells of each plane, loads the weights once, and runs the cells
+// through the network 16 per wavefront job, features read from LDS.
 // ---------------------------------------------------------------------------
 constexpr int kClZ = 8;
 
-__global__ __launch_bounds__(256) void k_closure(KParams p) {
+__global__ __launch_bounds__(256, 2) void k_closure(KParams p) {
     __shared__ MlpWeightsLds W;
+    constexpr int kPX = 66, kPY = 6;  // 64 x 4 tile with a 1-cell halo
+    __shared__ float s_ph[3][kPY * kPX];
     __shared__ int s_list[256];
     __shared__ int s_count[4];
     __shared__ int s_loaded;
@@ -320,9 +323,24 @@ __global__ __launch_bounds__(256) void k_closure(KParams p) {
         }
     }
     if (!__syncthreads_or(wants != 0)) return;  // uniform: no interface cell in this workgroup
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
+    auto stage = [&](int pz) {  // phi of plane pz, tile + halo, into ring slot pz % 3
+        float* dst = s_ph[pz % 3];
+        for (int i = tid; i < kPY * kPX; i += 256) {
+            const int gx = x0 - 1 + i % kPX, gy = y0 - 1 + i / kPX;
+            float v = 0.0f;
+            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny)
+                v = phi_g[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)pz)];
+            dst[i] = v;
+        }
+    };
+    const int zfirst = 1 + blockIdx.z * kClZ;
+    stage(zfirst - 1);
+    stage(zfirst);
     for (int k = 0; k < kClZ; ++k) {
-        const int z = 1 + blockIdx.z * kClZ + k;
+        const int z = zfirst + k;
         if (z > nz - 2) break;  // uniform over the workgroup
+        stage(z + 1);  // its slot held plane z - 2, last read before the previous barrier
         const bool want = (wants >> k) & 1u;
         const unsigned long long mask = __ballot(want);
         const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0));
@@ -346,26 +364,35 @@ __global__ __launch_bounds__(256) void k_closure(KParams p) {
             const int slot = 16 * job + i;
             const bool valid = slot < total;
             const int cell = valid ? s_list[slot] : 0;
-            const int cx = blockIdx.x * 64 + (cell & 63), cy = blockIdx.y * 4 + (cell >> 6);
-            auto ph = [&](int dx, int dy, int dz) {
-                return phi_g[(uint32_t)(cx + dx) + (uint32_t)nx * ((uint32_t)(cy + dy) + (uint32_t)ny * (uint32_t)(z + dz))];
-            };
+            const int cx = x0 + (cell & 63), cy = y0 + (cell >> 6);
+            const int o = ((cell >> 6) + 1) * kPX + (cell & 63) + 1;
+            auto ph = [&](int dx, int dy, int dz) { return s_ph[(z + dz) % 3][o + dy * kPX + dx]; };
             float pc = 0.0f, inv = 0.0f;
             if (valid) {
                 pc = ph(0, 0, 0);
                 inv = feat_scale(ph(-1, 0, 0), ph(1, 0, 0), ph(0, -1, 0), ph(0, 1, 0), ph(0, 0, -1), ph(0, 0, 1));
             }
-            bf16x8 xh, xl;
+            // The curvature of the point-reflected stencil (feature f <-> 26 - f)
+            // is the same; the closure averages the network over the stencil
+            // and its reflection, so the forces on opposite sides of a
+            // symmetric interface cancel exactly.
+            float yv = 0.0f;
 #pragma unroll
-            for (int r = 0; r < 8; ++r) {
-                const int f = 8 * g + r;
-                float v = 0.0f;
-                if (valid && f < kMlpIn)
-                    v = (f == kCentre) ? 2.0f * pc - 1.0f : (ph(feat_dx(f), feat_dy(f), feat_dz(f)) - pc) * inv;
-                xh[r] = (__bf16)v;
-                xl[r] = (__bf16)(v - (float)xh[r]);
+            for (int pass = 0; pass < 2; ++pass) {
+                const int sg = pass == 0 ? 1 : -1;
+                bf16x8 xh, xl;
+#pragma unroll
+                for (int r = 0; r < 8; ++r) {
+                    const int f = 8 * g + r;
+                    float v = 0.0f;
+                    if (valid && f < kMlpIn)
+                        v = (f == kCentre) ? 2.0f * pc - 1.0f
+                                           : (ph(sg * feat_dx(f), sg * feat_dy(f), sg * feat_dz(f)) - pc) * inv;
+                    xh[r] = (__bf16)v;
+                    xl[r] = (__bf16)(v - (float)xh[r]);
+                }
+                yv += 0.5f * mlp_eval16(W, xh, xl);
             }
-            const float yv = mlp_eval16(W, xh, xl);
             if (g == 0 && valid)
                 p.kappa[(uint32_t)cx + (uint32_t)nx * ((uint32_t)cy + (uint32_t)ny * (uint32_t)z)] = 0.25f * yv;
         }

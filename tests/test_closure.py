@@ -88,12 +88,14 @@ def test_stencil_features_planar_profile():
 
 
 def test_committed_closure_beats_fd_pointwise():
-    """On a sphere of radius 8 relaxed by the oracle, the network's curvature
+    """On a sphere of radius 8 relaxed by the oracle, the closure's curvature
     is within 2 % of -2/R on average and its rms error is at most a quarter
-    of the finite-difference one (measured: 0.8 % against 21 %)."""
+    of the finite-difference one (measured: 0.7 % against 21 %)."""
     phi, cells, k_fd = _relaxed_sphere(8, 40, 300)
     assert cells.size > 1000
-    k_mlp = 0.25 * ref.forward(closure.load(), closure.stencils(phi, cells), bf=True)[0]
+    X = closure.stencils(phi, cells)
+    p = closure.load()
+    k_mlp = 0.125 * (ref.forward(p, X, bf=True)[0] + ref.forward(p, X[:, ::-1], bf=True)[0])
     m_fd, r_fd = _rel(k_fd, 8)
     m_mlp, r_mlp = _rel(k_mlp, 8)
     assert abs(m_mlp) < 0.02, m_mlp
@@ -115,16 +117,19 @@ def test_closure_laplace():
     """Static droplets run to rest with the learned closure stay at rest and
     obey Laplace's law.  At R = 8, where the finite-difference curvature is
     least accurate, the pressure jump is closer to 2 sigma / R than with it
-    (measured: 0.5 % against 4.0 %); at R = 12 and 16 both are within a few
-    per cent (FD 0.3 % and 2.1 %, closure 1.9 % and 3.3 %).  The closure's
-    residual pointwise noise (about 1 % of kappa) raises the spurious
-    velocities from about 1e-4 to about 1e-3."""
+    (measured: 0.4 % against 4.0 %); at R = 12 and 16 both are within a few
+    per cent (FD 0.3 % and 2.1 %, closure 1.7 % and 3.1 %).  Spurious
+    velocities stay at the finite-difference level (measured 1.4e-4, 1.2e-4,
+    6.7e-5 against 2.2e-4, 9.0e-5, 6.1e-5): the closure averages the network
+    over each stencil and its point reflection, so the surface forces on
+    opposite sides of a symmetric droplet cancel (without it the droplet
+    drifted at ~1e-3)."""
     rows = closure.laplace(closure.load(), radii=(8, 12, 16), steps=3000)
     by_r = {r["R"]: r for r in rows}
     assert by_r[8]["mlp"]["rel_err"] < by_r[8]["fd"]["rel_err"], by_r[8]
     for r in rows:
         assert r["mlp"]["rel_err"] < 0.05, r
-        assert r["mlp"]["spurious_umax"] < 5e-3, r
+        assert r["mlp"]["spurious_umax"] < 3.0 * max(r["fd"]["spurious_umax"], 1e-4), r
 
 
 def test_raw_weights_match():

@@ -101,7 +101,7 @@ def test_mlp_training_fits():
 def test_closure_in_timestep():
     """k_closure + the closure variant of the fused step: the curvature the
     step applies at interface cells equals the reference network evaluated
-    on the same phi stencils."""
+    on the same phi stencils (and their point reflections)."""
     import phasefieldlbm_amd as pf
     from phasefieldlbm_amd import closure, presets
     rng = np.random.default_rng(5)
@@ -121,7 +121,10 @@ def test_closure_in_timestep():
     sel = inner & (phi > cfg.mlp_lo) & (phi < cfg.mlp_hi) & (ind > 1e-3)
     cells = np.flatnonzero(sel.ravel())
     assert cells.size > 100
-    y, _ = ref.forward(p, closure.stencils(phi, cells), bf=True)
+    # the closure averages the network over the stencil and its point
+    # reflection (feature f <-> 26 - f)
+    X = closure.stencils(phi, cells)
+    y = 0.5 * (ref.forward(p, X, bf=True)[0] + ref.forward(p, X[:, ::-1], bf=True)[0])
     k_ref = 0.25 * y
     n2 = (nv ** 2).sum(0).ravel()[cells]
     k_gpu = -(fs * nv).sum(0).ravel()[cells] / (cfg.sigma * ind.ravel()[cells] * n2)

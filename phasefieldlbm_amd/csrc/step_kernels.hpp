@@ -614,7 +614,11 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             const float uF = ux * fsx + uy * fsy + uz * fsz;
             Coll k;
             k.V = 1.0f - 1.5f * usq;
-            k.B0 = rho * k.V - 1.0f - 1.5f * uF;
+            // rho V - 1 - 1.5 u.F with rho - 1 taken as the population sum m0:
+            // rho = m0 + 1 rounds the density deviation to a multiple of
+            // ulp(1) ~ 1.2e-7, the same way in every cell of a uniform
+            // region, which would bias the mass by that much per step.
+            k.B0 = m0 - 1.5f * (rho * usq + uF);
             k.R3 = 3.0f * rho;
             k.R45 = 4.5f * rho;
             k.O45 = 4.5f * omco;

@@ -1,6 +1,7 @@
 """Run a named workload for a few timesteps, for profilers.
 
     python -m phasefieldlbm_amd.profile_step --workload jet_d3q27_256x256x1024 --steps 4
+    python -m phasefieldlbm_amd.profile_step --workload jet_d3q27_256x256x1024 --closure --warmup 3000 --steps 20
     python -m phasefieldlbm_amd.profile_step --copy-bytes 1073741824
 
 The first form initialises the workload and runs --steps timesteps (an even
@@ -26,6 +27,7 @@ def main(argv=None):
     ap.add_argument("--zchunk", type=int, default=0)
     ap.add_argument("--slot-pad", type=int, default=-1)
     ap.add_argument("--copy-bytes", type=int, default=0)
+    ap.add_argument("--closure", action="store_true", help="learned curvature closure (committed weights)")
     args = ap.parse_args(argv)
 
     import phasefieldlbm_amd as pf
@@ -38,7 +40,12 @@ def main(argv=None):
         make, _ = presets.WORKLOADS[args.workload]
         cfg = make()
         cfg.zchunk, cfg.slot_pad = args.zchunk, args.slot_pad
+        if args.closure:
+            cfg.closure = pf.CLOSURE_MLP
         with pf.Simulation(cfg) as sim:
+            if args.closure:
+                from phasefieldlbm_amd import closure
+                sim.set_closure_params(closure.load())
             sim.init().step(args.warmup)
             sim.sync()
             print(f"{args.workload}: warmup {args.warmup} done, timing {args.steps} steps", flush=True)

@@ -133,11 +133,29 @@ def phi_field(axes, rot, centre, n=BOX):
     return phi
 
 
-def curvature_labels(axes, rot, centre, cells, n=BOX):
-    """kappa = 2 H at the point of the ellipsoid closest to each cell centre."""
+def gauss_curvature(axes, x):
+    """Gaussian curvature of the ellipsoid at surface points x (body frame)."""
+    a2 = axes ** 2
+    return 1.0 / (np.prod(a2) * ((x * x / (a2 * a2)).sum(-1)) ** 2)
+
+
+def curvature_labels(axes, rot, centre, cells, n=BOX, label="surface"):
+    """Curvature label of each cell, in the reference's sign (div n, -2/R on
+    a sphere).  "surface": 2 H at the point of the ellipsoid closest to the
+    cell centre.  "levelset": the curvature of the level set through the
+    cell, at signed distance d from the surface (d > 0 outside): the sum of
+    c_i / (1 - d c_i) over the principal curvatures c_i at the closest point
+    (-2/r on a sphere), the quantity finite-difference div n approximates."""
     z, y, x = np.unravel_index(cells, (n, n, n))
     p = _body(np.stack([x, y, z], -1).astype(np.float64), rot, centre)
-    return mean_curvature2(axes, closest_point(axes, p)[0]).astype(np.float32)
+    xs, d = closest_point(axes, p)
+    k2 = mean_curvature2(axes, xs)
+    if label == "surface":
+        return k2.astype(np.float32)
+    K = gauss_curvature(axes, xs)
+    disc = np.sqrt(np.maximum(k2 * k2 - 4.0 * K, 0.0))
+    c1, c2 = 0.5 * (k2 + disc), 0.5 * (k2 - disc)
+    return (c1 / (1.0 - d * c1) + c2 / (1.0 - d * c2)).astype(np.float32)
 
 
 def ellipsoid_fields(axes, rot, centre, n=BOX):
@@ -175,7 +193,7 @@ def relax_config():
     return presets.static_droplet(n=BOX, radius=8.0, velocity_set=D3Q19, sigma=0.0, tau_g=TAU_G)
 
 
-def make_dataset(n_drops=4096, steps=500, per_drop=256, seed=7, concurrent=16, log=print):
+def make_dataset(n_drops=4096, steps=500, per_drop=256, seed=7, concurrent=16, log=print, label="surface"):
     """(X [n,27], T [n]) from n_drops relaxed droplets/ellipsoids.  Shapes
     are drawn up front; their phase fields are built on a thread pool while
     `concurrent` simulations, each on its own stream, relax earlier ones."""
@@ -210,7 +228,7 @@ def make_dataset(n_drops=4096, steps=500, per_drop=256, seed=7, concurrent=16, l
                 if cand.size > per_drop:
                     cand = rng.choice(cand, per_drop, replace=False)
                 xs.append(stencils(phi, cand))
-                ts.append(4.0 * curvature_labels(*shape, cand))
+                ts.append(4.0 * curvature_labels(*shape, cand, label=label))
             done += len(batch)
             if done % 512 == 0 or done == n_drops:
                 log(f"[closure] {done}/{n_drops} droplets relaxed, {time.time() - t0:.1f} s")
@@ -326,6 +344,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--eval", metavar="NPZ", help="only evaluate saved weights")
+    ap.add_argument("--label", choices=("surface", "levelset"), default="surface",
+                    help="curvature label: sharp surface at the closest point, or the level set through the cell")
     args = ap.parse_args(argv)
     t0 = time.time()
     if args.eval:
@@ -333,7 +353,7 @@ def main(argv=None):
         print(json.dumps({"curvature": curvature_error(params)}), flush=True)
         print(json.dumps({"laplace": laplace(params), "seconds": time.time() - t0}), flush=True)
         return
-    X, T = make_dataset(args.drops, args.steps, args.per_drop)
+    X, T = make_dataset(args.drops, args.steps, args.per_drop, label=args.label)
     rng = np.random.default_rng(11)
     perm = rng.permutation(T.size)
     nval = T.size // 20

@@ -67,6 +67,38 @@ def test_closest_point_projection():
     np.testing.assert_allclose(ks, -0.2, rtol=1e-6)
 
 
+def test_curvature_labels_closed_forms():
+    """Both training labels against closed forms.  Sphere of radius R: the
+    surface label is -2/R and the level-set label -2/r at distance r from
+    the centre.  Ellipsoid (a, b, c) on the x axis at signed distance d from
+    the tip (a, 0, 0), where the principal curvatures are -a/b^2 and -a/c^2:
+    the surface label is their sum at every d, and the level-set label is
+    -1/(b^2/a + d) - 1/(c^2/a + d), equal to the surface label at d = 0."""
+    n = closure.BOX
+    rng = np.random.default_rng(3)
+    R, centre = 10.0, np.full(3, 31.5)
+    cells = rng.choice(n ** 3, 4000, replace=False)
+    z, y, x = np.unravel_index(cells, (n,) * 3)
+    r = np.linalg.norm(np.stack([x, y, z], -1) - centre, axis=1)
+    keep = np.abs(r - R) < 3.0
+    cells, r = cells[keep], r[keep]
+    assert cells.size > 20
+    axes = np.full(3, R)
+    np.testing.assert_allclose(closure.curvature_labels(axes, np.eye(3), centre, cells), -2.0 / R, rtol=1e-5)
+    np.testing.assert_allclose(closure.curvature_labels(axes, np.eye(3), centre, cells, label="levelset"),
+                               -2.0 / r, rtol=1e-5)
+
+    a, b, c = 14.0, 9.0, 6.0
+    centre = np.full(3, 32.0)
+    d = np.arange(-2, 4, dtype=np.float64)  # |d| below the tip's radii of curvature (2.57, 5.79)
+    cells = np.ravel_multi_index((np.full(d.size, 32), np.full(d.size, 32), (32 + a + d).astype(int)), (n,) * 3)
+    surf = closure.curvature_labels(np.array([a, b, c]), np.eye(3), centre, cells)
+    ls = closure.curvature_labels(np.array([a, b, c]), np.eye(3), centre, cells, label="levelset")
+    np.testing.assert_allclose(surf, -a / b ** 2 - a / c ** 2, rtol=1e-5)
+    np.testing.assert_allclose(ls, -1.0 / (b * b / a + d) - 1.0 / (c * c / a + d), rtol=1e-5)
+    assert ls[d == 0] == pytest.approx(surf[d == 0], rel=1e-6)
+
+
 def test_stencil_features_planar_profile():
     """Normalised features of a planar tanh profile along x: the centre is
     2 phi - 1, the y/z neighbours in the cell's own plane are zero, and the

@@ -33,8 +33,9 @@ def log(msg):
 
 def kernel_bytes_per_cell(q, s):
     """Algorithmic bytes of the fused collide-stream kernel per interior cell:
-    read and write Q f populations, write 7 g populations, read phi."""
-    return 2 * q * s + 7 * s + 4
+    read and write Q f populations, write the 3-component FP32 phase flux,
+    read phi."""
+    return 2 * q * s + 12 + 4
 
 
 def run_form(pf, cfg, steps, warmup, closure_params=None):
@@ -115,6 +116,7 @@ def main():
     ms = wall * 1e3 / args.steps
     mlups = cells * args.steps / wall / 1e6
     b_alg = presets.bytes_per_update(cfg)
+    b_moved = presets.bytes_moved_per_update(cfg)
     step_ms = tm["step_ms"] / args.steps
     kbytes = kernel_bytes_per_cell(cfg.velocity_set, 4) * inner
     achieved = kbytes / (step_ms * 1e-3) / 1e9
@@ -154,6 +156,8 @@ def main():
         "measured_copy_gbs": round(copy_gbs, 1),
         "step_frac_of_copy": round(mlups * 1e6 * b_alg / (copy_gbs * 1e9), 4),
         "bytes_per_update": b_alg,
+        "bytes_moved_per_update": b_moved,
+        "moved_roofline_frac": round(mlups * 1e6 * b_moved / (HBM_PEAK_GBS * 1e9), 4),
         "roofline": {"bound": "hbm", "kernel": "k_step", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": kbytes, "avg_launch_ms": round(step_ms, 4)},

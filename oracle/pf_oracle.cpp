@@ -5,8 +5,9 @@
 // checker for the HIP library: only tests/, __graft_entry__.smoke() and
 // bench.py's cpu_baseline leg may load it.  Nothing in the product links it.
 //
-// Instantiations: FP32 (parity oracle), FP32 arithmetic with binary16
-// population storage (precision-form oracle) and FP64 (accuracy oracle).
+// Instantiations: FP32 (parity oracle), FP32 arithmetic with binary16 f
+// storage (precision-form oracle; g stays FP32 as in the reference's
+// precision.cuh) and FP64 (accuracy oracle).
 //
 // Reference files restated here (paths relative to the reference root):
 //   src/phaseField.cuh:31-219        computePhase / computeNormals / computeForces
@@ -267,7 +268,8 @@ struct Sim final : Base {
     int nx, ny, nz;
     size_t N;
     std::vector<T> fld[PF_NUM_FIELDS];
-    std::vector<S> f, g;
+    std::vector<S> f;
+    std::vector<T> g;
     bool jet;
 
     explicit Sim(const pf_config& cfg) : c(cfg) {
@@ -326,18 +328,8 @@ struct Sim final : Base {
         if constexpr (HALF) f[(size_t)q * N + i] = f2h((float)v);
         else f[(size_t)q * N + i] = v;
     }
-    // binary16 storage keeps g_q of cell i relative to w_q * phi(i), the phi
-    // field of the step that stored it (the destination cell's phi): the
-    // stored numbers are the small velocity and gradient parts of g, so
-    // their rounding carries no systematic bias from the bulk value w_q*phi.
-    T ld_g(int q, size_t i) const {
-        if constexpr (HALF) return (T)(h2f(g[(size_t)q * N + i]) + (float)G.w[q] * (float)fld[PF_FIELD_PHI][i]);
-        else return g[(size_t)q * N + i];
-    }
-    void st_g(int q, size_t i, T v) {
-        if constexpr (HALF) g[(size_t)q * N + i] = f2h((float)v - (float)G.w[q] * (float)fld[PF_FIELD_PHI][i]);
-        else g[(size_t)q * N + i] = v;
-    }
+    T ld_g(int q, size_t i) const { return g[(size_t)q * N + i]; }
+    void st_g(int q, size_t i, T v) { g[(size_t)q * N + i] = v; }
 
     T& F(int id, size_t i) { return fld[id][i]; }
 
@@ -588,14 +580,9 @@ struct Sim final : Base {
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 const T dx = (T)x - k.cx, dy = (T)y - k.cy;
-                if (dx * dx + dy * dy > k.R2) {
-                    // Frozen wall: no population reaches z = 1 from z = 0 off the
-                    // nozzle, so g_5 there keeps its initial value 0.  binary16
-                    // storage is relative to the current phi of the cell, so the
-                    // zero is stored again each step.
-                    if (HALF && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2) st_g(5, id3(x, y, 1), (T)0);
-                    continue;
-                }
+                // Frozen wall: no population reaches z = 1 from z = 0 off the
+                // nozzle, so g_5 there keeps its initial value 0.
+                if (dx * dx + dy * dy > k.R2) continue;
                 const size_t ib = id3(x, y, 0);
                 T ux = 0, uy = 0, uz = k.u_inf;
                 if (c.noise == PF_NOISE_GAUSSIAN_XY) {

@@ -2,16 +2,18 @@
 //
 //   k_init_fields    setInitialDensity + setJet (src/initialConditions.cu:33-99);
 //                    the droplet profile is computed on the host
-//   k_init_distros   setDistros (src/initialConditions.cu:101-138), written in
-//                    the layout the first (even) A-A step reads
+//   k_init_distros   setDistros (src/initialConditions.cu:101-138) for f,
+//                    written in the layout the first (even) A-A step reads;
+//                    the first phase sum forms the equilibrium g itself
 //   k_gather         post-streaming populations in the reference layout
 //   k_diagnostics    FP64 sums of rho, rho*u, phi and max|u| (no reference
 //                    counterpart; the reference prints no diagnostics)
-//   k_phase_mass     FP64 sum of phi = sum_q g_q straight from the populations
+//   k_phase_mass     FP64 sum of phi = sum_q g_q over the arriving populations
 #include <hip/hip_runtime.h>
 
 #include "kparams.hpp"
 #include "lattice.hpp"
+#include "phase_flux.hpp"
 
 namespace pf {
 
@@ -58,7 +60,6 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= p.N) return;
     const float rho = p.fld[0][i], ux = p.fld[1][i], uy = p.fld[2][i], uz = p.fld[3][i];
-    const float phi = p.fld[10][i];
     const float uu = 1.5f * (ux * ux + uy * uy + uz * uz);
     const int x = (int)(i % (uint32_t)p.nx);
     const int y = (int)((i / (uint32_t)p.nx) % (uint32_t)p.ny);
@@ -67,16 +68,11 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
         constexpr int q = decltype(qc)::value;
         st_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, p.nx, q, x, y, z), f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
     });
-    static_for<7>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        float v = g_post<q>(phi, ux, uy, uz, 0.0f, 0.0f, 0.0f, 0.0f);
-        if constexpr (HALF) v -= wq<7>(q) * phi;  // binary16: relative to w_q * phi of the cell
-        st_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, p.nx, q, x, y, z), v);
-    });
 }
 
-// which = 0: f (Q slots, stored shift -w kept); which = 1: g (7 slots, unshifted)
-template <int Q, bool JET, bool HALF, int PAR>
+// which = 0: f (Q slots, stored shift -w kept); which = 1: g (7 slots, the
+// populations the next phase sum adds)
+template <int Q, bool JET, bool HALF, int PAR, bool LOCAL>
 __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= p.N) return;
@@ -93,8 +89,13 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
         zp = (z == nz - 2) ? 1 : z + 1;
     }
     const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp}, zs[3] = {zm, z, zp};
-    const int nq = which == 0 ? Q : 7;
-    for (int q = 0; q < nq; ++q) {
+    if (which == 1) {
+        float g[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (interior) arriving_g<JET, LOCAL>(p, x, y, z, g);
+        for (int q = 0; q < 7; ++q) out[(size_t)q * p.N + i] = g[q];
+        return;
+    }
+    for (int q = 0; q < Q; ++q) {
         float v = 0.0f;
         if (interior) {
             int sx = x, sy = y, sz = z, slot = q;
@@ -104,12 +105,7 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
                 sy = ys[1 - kCY[q]];
                 sz = zs[1 - kCZ[q]];
             }
-            if (which == 0) {
-                v = ld_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, nx, slot, sx, sy, sz));
-            } else {
-                v = ld_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, nx, slot, sx, sy, sz));
-                if constexpr (HALF) v += wq<7>(q) * p.fld[10][i];
-            }
+            v = ld_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, nx, slot, sx, sy, sz));
         }
         out[(size_t)q * p.N + i] = v;
     }
@@ -165,7 +161,7 @@ __global__ __launch_bounds__(256) void k_diagnostics(KParams p, double* partials
     block_reduce<6, true>(v, partials + (size_t)blockIdx.x * 6);
 }
 
-template <bool JET, bool HALF, int PAR>
+template <bool JET, bool LOCAL>
 __global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials) {
     double v[1] = {0.0};
     const int nx = p.nx, ny = p.ny, nz = p.nz;
@@ -174,27 +170,10 @@ __global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials)
         const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
         const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
         if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
-        const int xm = (x == 1) ? nx - 2 : x - 1, xp = (x == nx - 2) ? 1 : x + 1;
-        const int ym = (y == 1) ? ny - 2 : y - 1, yp = (y == ny - 2) ? 1 : y + 1;
-        int zm = z - 1, zp = z + 1;
-        if constexpr (!JET) {
-            zm = (z == 1) ? nz - 2 : z - 1;
-            zp = (z == nz - 2) ? 1 : z + 1;
-        }
-        const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp}, zs[3] = {zm, z, zp};
+        float g[7];
+        arriving_g<JET, LOCAL>(p, x, y, z, g);
         double phi = 0.0;
-        for (int q = 0; q < 7; ++q) {
-            int sx = x, sy = y, sz = z, slot = q;
-            if constexpr (PAR == 1) {
-                slot = opp(q);
-                sx = xs[1 - kCX[q]];
-                sy = ys[1 - kCY[q]];
-                sz = zs[1 - kCZ[q]];
-            }
-            float gv = ld_pop<HALF>(p.g, pop_elem(p.plane_g, p.pitch_g, nx, slot, sx, sy, sz));
-            if constexpr (HALF) gv += wq<7>(q) * p.fld[10][i];
-            phi += (double)gv;
-        }
+        for (int q = 0; q < 7; ++q) phi += (double)g[q];
         v[0] += phi;
     }
     block_reduce<1, false>(v, partials + blockIdx.x);
@@ -261,22 +240,28 @@ hipError_t launch_init_distros(const KParams& p, int Q, bool half, hipStream_t s
 }
 
 template <int Q, bool JET, bool HALF>
-static hipError_t gather_t(const KParams& p, int parity, int which, float* out, hipStream_t s) {
+static hipError_t gather_t(const KParams& p, int parity, bool local, int which, float* out, hipStream_t s) {
     const dim3 grid((p.N + 255) / 256);
-    if (parity) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 1>), grid, dim3(256), 0, s, p, which, out);
-    else hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0>), grid, dim3(256), 0, s, p, which, out);
+    if (local) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0, true>), grid, dim3(256), 0, s, p, which, out);
+    else if (parity) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 1, false>), grid, dim3(256), 0, s, p, which, out);
+    else hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0, false>), grid, dim3(256), 0, s, p, which, out);
     return hipGetLastError();
 }
 
 template <int Q>
-static hipError_t gather_q(const KParams& p, bool jet, bool half, int parity, int which, float* out, hipStream_t s) {
-    if (jet) return half ? gather_t<Q, true, true>(p, parity, which, out, s) : gather_t<Q, true, false>(p, parity, which, out, s);
-    return half ? gather_t<Q, false, true>(p, parity, which, out, s) : gather_t<Q, false, false>(p, parity, which, out, s);
+static hipError_t gather_q(const KParams& p, bool jet, bool half, int parity, bool local, int which, float* out,
+                           hipStream_t s) {
+    if (jet)
+        return half ? gather_t<Q, true, true>(p, parity, local, which, out, s)
+                    : gather_t<Q, true, false>(p, parity, local, which, out, s);
+    return half ? gather_t<Q, false, true>(p, parity, local, which, out, s)
+                : gather_t<Q, false, false>(p, parity, local, which, out, s);
 }
 
-hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, int which, float* out,
-                         hipStream_t s) {
-    return Q == 27 ? gather_q<27>(p, jet, half, parity, which, out, s) : gather_q<19>(p, jet, half, parity, which, out, s);
+hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, bool local, int which,
+                         float* out, hipStream_t s) {
+    return Q == 27 ? gather_q<27>(p, jet, half, parity, local, which, out, s)
+                   : gather_q<19>(p, jet, half, parity, local, which, out, s);
 }
 
 // Running means of the derived-field set over every cell, ghosts included
@@ -343,17 +328,15 @@ hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, h
     return hipGetLastError();
 }
 
-template <bool JET, bool HALF>
-static hipError_t mass_t(const KParams& p, int parity, double* partials, int nblocks, hipStream_t s) {
-    if (parity) hipLaunchKernelGGL((k_phase_mass<JET, HALF, 1>), dim3(nblocks), dim3(256), 0, s, p, partials);
-    else hipLaunchKernelGGL((k_phase_mass<JET, HALF, 0>), dim3(nblocks), dim3(256), 0, s, p, partials);
+template <bool JET>
+static hipError_t mass_t(const KParams& p, bool local, double* partials, int nblocks, hipStream_t s) {
+    if (local) hipLaunchKernelGGL((k_phase_mass<JET, true>), dim3(nblocks), dim3(256), 0, s, p, partials);
+    else hipLaunchKernelGGL((k_phase_mass<JET, false>), dim3(nblocks), dim3(256), 0, s, p, partials);
     return hipGetLastError();
 }
 
-hipError_t launch_phase_mass(const KParams& p, bool jet, bool half, int parity, double* partials, int nblocks,
-                             hipStream_t s) {
-    if (jet) return half ? mass_t<true, true>(p, parity, partials, nblocks, s) : mass_t<true, false>(p, parity, partials, nblocks, s);
-    return half ? mass_t<false, true>(p, parity, partials, nblocks, s) : mass_t<false, false>(p, parity, partials, nblocks, s);
+hipError_t launch_phase_mass(const KParams& p, bool jet, bool local, double* partials, int nblocks, hipStream_t s) {
+    return jet ? mass_t<true>(p, local, partials, nblocks, s) : mass_t<false>(p, local, partials, nblocks, s);
 }
 
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s) {

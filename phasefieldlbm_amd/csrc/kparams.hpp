@@ -12,12 +12,25 @@
 //         src/functions/deviceFunctions.cuh:33-48, made timings depend on
 //         where the driver placed the physical pages).  Addressed in place
 //         with the A-A pattern (see the step kernels).
-//   g   : 7 slots, same storage type and layout (pitch_g >= 7*nx)
+//   h   : 3 FP32 arrays [N], the phase flux of every cell,
+//         h = (4 phi u + gamma phi (1 - phi) n) / 8.  The D3Q7 phase
+//         populations have no non-equilibrium part (D3Q7.cuh:139-143), so the
+//         post-collision g of a cell is fixed by its phi and h:
+//         g_0 = phi/4, g_(+-k) = phi/8 +- h_k.  The phase kernel rebuilds the
+//         seven g arriving at each cell from its neighbours' phi and h instead
+//         of storing and streaming the g (12 B written per cell instead of
+//         7 populations).
 //   fld : 18 FP32 macroscopic fields [N] in host::FieldID order; phi is
 //         live every step, the others are written on save steps only
+//   phi_next : the second phi buffer: the phase kernel reads phi of the
+//         previous step from fld[10] and writes the new phi here; the host
+//         then swaps the two
 //   bcp : jet boundary planes, 2 planes x 13 values x nx*ny (rho, u, P, F at
 //         z = 1 and z = nz-2), written by the fused step for the boundary
 //         kernel
+//   gbc : jet boundary phase populations, 2 planes x nx*ny: the g_5 entering
+//         z = 1 and the g_6 entering z = nz-2 (src/BoundaryConditions.cuh:104,
+//         168)
 // idx = x + nx*(y + ny*z), x fastest; the outer 1-cell shell is a ghost layer.
 #pragma once
 
@@ -53,14 +66,16 @@ constexpr int kMlpHidden = 64;
 
 struct KParams {
     void* f;
-    void* g;
+    float* h[3];
     float* fld[kNumFields];
+    float* phi_next;
     float* bcp;
+    float* gbc;
     int nx, ny, nz;
     uint32_t N;  // cells
-    uint32_t pitch_f, pitch_g;  // population row pitch in elements (y stride); slot stride is nx
-    uint64_t plane_f, plane_g;  // population plane pitch in elements (z stride) = pitch * ny
-    uint64_t elems_f, elems_g;  // population array sizes in elements = plane * nz
+    uint32_t pitch_f;  // population row pitch in elements (y stride); slot stride is nx
+    uint64_t plane_f;  // population plane pitch in elements (z stride) = pitch * ny
+    uint64_t elems_f;  // population array size in elements = plane * nz
     int zchunk;
     int ntx, nty, ntz;  // tile counts
     int save;           // write every macroscopic field this step
@@ -85,7 +100,9 @@ struct KParams {
 };
 
 // Launchers; all return hipError_t of the launch.
-hipError_t launch_phase_sum(const KParams& p, int Q, bool jet, bool half, int parity, hipStream_t s);
+// local: the first step after the populations were set, when each cell's
+// own equilibrium g (not its neighbours' post-collision g) is summed
+hipError_t launch_phase_sum(const KParams& p, bool jet, bool local, hipStream_t s);
 // variant: 0 pair-symmetric collision, 1 literal per-direction evaluation,
 // 2 pair-symmetric with the learned curvature closure
 hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, int variant, hipStream_t s);
@@ -93,11 +110,10 @@ hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hi
 hipError_t launch_init_fields(const KParams& p, bool jet, float radius, float ax, float ay, float az, float W,
                               hipStream_t s);
 hipError_t launch_init_distros(const KParams& p, int Q, bool half, hipStream_t s);
-hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, int which, float* out,
-                         hipStream_t s);
+hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, bool local, int which,
+                         float* out, hipStream_t s);
 hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, hipStream_t s);
-hipError_t launch_phase_mass(const KParams& p, bool jet, bool half, int parity, double* partials, int nblocks,
-                             hipStream_t s);
+hipError_t launch_phase_mass(const KParams& p, bool jet, bool local, double* partials, int nblocks, hipStream_t s);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
 

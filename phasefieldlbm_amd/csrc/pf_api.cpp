@@ -25,13 +25,15 @@ struct pf_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
     size_t N = 0;
-    size_t f_elems = 0, g_elems = 0;  // population array sizes (elements, row-interleaved layout)
+    size_t f_elems = 0;  // population array size (elements, row-interleaved layout)
     int Q = 19;
     bool jet = true, half = false;
     void* f = nullptr;
-    void* g = nullptr;
+    float* h = nullptr;  // phase flux, 3 x N
     float* fld[pf::kNumFields] = {};
+    float* phi_next = nullptr;  // second phi buffer, swapped with fld[PF_FIELD_PHI] every step
     float* bcp = nullptr;
+    float* gbc = nullptr;  // jet inflow g_5 / outflow g_6 planes
     double* partials = nullptr;
     double* dout = nullptr;
     int nred = 0;
@@ -43,6 +45,7 @@ struct pf_ctx {
     bool has_mlp = false;
     pf::KParams kp{};
     int parity = 0;
+    bool fresh = false;  // populations just set: the next phase sum is the local one
     bool inited = false;
     size_t bytes = 0;
     std::string err;
@@ -254,14 +257,13 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     // elements (default none), plane pitch = row pitch * ny.  In-plane byte
     // offsets are 32-bit in the kernels.
     const uint64_t pad = g.slot_pad >= 0 ? (uint64_t)g.slot_pad : 0;
-    const uint64_t pitch_f = (uint64_t)c->Q * g.nx + pad, pitch_g = (uint64_t)7 * g.nx + pad;
+    const uint64_t pitch_f = (uint64_t)c->Q * g.nx + pad;
     const uint64_t esize = c->half ? 2 : 4;
     if (pitch_f * g.ny * esize >= ((uint64_t)1 << 32)) {
         delete c;
         return PF_ERR_CONFIG;
     }
     c->f_elems = (size_t)(pitch_f * g.ny * g.nz);
-    c->g_elems = (size_t)(pitch_g * g.ny * g.nz);
 
     // device selection (src/functions/hostFunctions.cuh:117-150)
     int dev = g.device;
@@ -296,10 +298,12 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
         c->bytes += bytes;
         return true;
     };
-    bool ok = alloc(&c->f, c->f_elems * pop_bytes(c)) && alloc(&c->g, c->g_elems * pop_bytes(c));
+    bool ok = alloc(&c->f, c->f_elems * pop_bytes(c)) && alloc((void**)&c->h, 3 * N * sizeof(float));
     for (int i = 0; ok && i < pf::kNumFields; ++i) ok = alloc((void**)&c->fld[i], N * sizeof(float));
+    ok = ok && alloc((void**)&c->phi_next, N * sizeof(float));
     const size_t plane = (size_t)g.nx * g.ny;
     ok = ok && alloc((void**)&c->bcp, 2 * pf::kPlaneVals * plane * sizeof(float));
+    ok = ok && alloc((void**)&c->gbc, 2 * plane * sizeof(float));
     c->nred = 1024;
     ok = ok && alloc((void**)&c->partials, (size_t)c->nred * 6 * sizeof(double));
     ok = ok && alloc((void**)&c->dout, 8 * sizeof(double));
@@ -318,20 +322,19 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
 
     pf::KParams& k = c->kp;
     k.f = c->f;
-    k.g = c->g;
+    for (int i = 0; i < 3; ++i) k.h[i] = c->h + (size_t)i * N;
     for (int i = 0; i < pf::kNumFields; ++i) k.fld[i] = c->fld[i];
+    k.phi_next = c->phi_next;
     k.bcp = c->bcp;
+    k.gbc = c->gbc;
     k.kappa = c->kappa;
     k.nx = g.nx;
     k.ny = g.ny;
     k.nz = g.nz;
     k.N = (uint32_t)N;
     k.pitch_f = (uint32_t)pitch_f;
-    k.pitch_g = (uint32_t)pitch_g;
     k.plane_f = pitch_f * (uint64_t)g.ny;
-    k.plane_g = pitch_g * (uint64_t)g.ny;
     k.elems_f = c->f_elems;
-    k.elems_g = c->g_elems;
     k.zchunk = choose_zchunk(g);
     k.ntx = (g.nx + pf::kTX - 1) / pf::kTX;
     k.nty = (g.ny + pf::kTY - 1) / pf::kTY;
@@ -346,9 +349,11 @@ void pf_destroy(pf_ctx* c) {
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     (void)hipFree(c->f);
-    (void)hipFree(c->g);
+    (void)hipFree(c->h);
     for (auto* p : c->fld) (void)hipFree(p);
+    (void)hipFree(c->phi_next);
     (void)hipFree(c->bcp);
+    (void)hipFree(c->gbc);
     (void)hipFree(c->partials);
     (void)hipFree(c->dout);
     (void)hipFree(c->gather_buf);
@@ -368,7 +373,13 @@ int pf_init_distros(pf_ctx* c) {
     PF_HIP(c, pf::launch_init_distros(c->kp, c->Q, c->half, c->stream));
     const size_t plane = (size_t)c->cfg.nx * c->cfg.ny;
     PF_HIP(c, hipMemsetAsync(c->bcp, 0, 2 * pf::kPlaneVals * plane * sizeof(float), c->stream));
+    PF_HIP(c, hipMemsetAsync(c->gbc, 0, 2 * plane * sizeof(float), c->stream));
+    PF_HIP(c, hipMemsetAsync(c->h, 0, 3 * c->N * sizeof(float), c->stream));
+    // the ghost shell of phi is read from whichever buffer is current
+    PF_HIP(c, hipMemcpyAsync(c->phi_next, c->fld[PF_FIELD_PHI], c->N * sizeof(float), hipMemcpyDeviceToDevice,
+                             c->stream));
     c->parity = 0;
+    c->fresh = true;
     c->inited = true;
     return PF_OK;
 }
@@ -382,7 +393,6 @@ int pf_init(pf_ctx* c) {
     for (auto* d : c->der)
         if (d) PF_HIP(c, hipMemsetAsync(d, 0, c->N * sizeof(float), c->stream));
     PF_HIP(c, hipMemsetAsync(c->f, 0, c->f_elems * pop_bytes(c), c->stream));
-    PF_HIP(c, hipMemsetAsync(c->g, 0, c->g_elems * pop_bytes(c), c->stream));
     PF_HIP(c, pf::launch_init_fields(c->kp, c->jet, g.radius, g.axis_x, g.axis_y, g.axis_z, g.interface_width,
                                      c->stream));
     if (!c->jet) {
@@ -429,6 +439,13 @@ static pf::DerivedPtrs derived_ptrs(const pf_ctx* c) {
     return d;
 }
 
+// The phase sum wrote the new phi into phi_next: make it the current field.
+static void swap_phi(pf_ctx* c) {
+    std::swap(c->fld[PF_FIELD_PHI], c->phi_next);
+    c->kp.fld[PF_FIELD_PHI] = c->fld[PF_FIELD_PHI];
+    c->kp.phi_next = c->phi_next;
+}
+
 static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hipEvent_t* ev) {
     // Running means need the macroscopic fields of every step (the reference
     // computes them every step and averages after the boundary conditions).
@@ -439,7 +456,9 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
         c->kp.save = (avg || ((flags & PF_STEP_SAVE_LAST) && s + 1 == nsteps)) ? 1 : 0;
         const int variant = c->cfg.closure == PF_CLOSURE_MLP ? 2 : c->cfg.kernel;
         if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
-        PF_HIP(c, pf::launch_phase_sum(c->kp, c->Q, c->jet, c->half, c->parity, c->stream));
+        PF_HIP(c, pf::launch_phase_sum(c->kp, c->jet, c->fresh, c->stream));
+        swap_phi(c);
+        c->fresh = false;
         if (variant == 2) PF_HIP(c, pf::launch_closure(c->kp, c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[1], c->stream));
         PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
@@ -499,7 +518,8 @@ int pf_get_populations(pf_ctx* c, int32_t which, float* host, size_t n) {
         }
         c->bytes += (size_t)c->Q * c->N * sizeof(float);
     }
-    PF_HIP(c, pf::launch_gather(c->kp, c->Q, c->jet, c->half, c->parity, which, c->gather_buf, c->stream));
+    PF_HIP(c, pf::launch_gather(c->kp, c->Q, c->jet, c->half, c->parity, c->fresh, which, c->gather_buf,
+                                c->stream));
     PF_HIP(c, hipMemcpyAsync(host, c->gather_buf, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     PF_HIP(c, hipStreamSynchronize(c->stream));
     return PF_OK;
@@ -521,7 +541,7 @@ int pf_phase_mass(pf_ctx* c, double* mass) {
     if (!c || !mass) return PF_ERR_CONFIG;
     if (!c->inited) return fail(c, PF_ERR_STATE, "phase mass before pf_init");
     PF_HIP(c, hipSetDevice(c->dev));
-    PF_HIP(c, pf::launch_phase_mass(c->kp, c->jet, c->half, c->parity, c->partials, c->nred, c->stream));
+    PF_HIP(c, pf::launch_phase_mass(c->kp, c->jet, c->fresh, c->partials, c->nred, c->stream));
     PF_HIP(c, pf::launch_reduce_partials(c->partials, c->nred, 1, c->dout, c->stream));
     PF_HIP(c, hipMemcpyAsync(mass, c->dout, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PF_HIP(c, hipStreamSynchronize(c->stream));

@@ -8,13 +8,9 @@ hipError_t launch_step_q27(const KParams& p, bool jet, bool half, int parity, in
 hipError_t launch_jet_bc_q19(const KParams& p, bool half, int next_parity, hipStream_t s);
 hipError_t launch_jet_bc_q27(const KParams& p, bool half, int next_parity, hipStream_t s);
 
-hipError_t launch_phase_sum(const KParams& p, int, bool jet, bool half, int parity, hipStream_t s) {
-    if (jet) {
-        if (half) return parity ? phase_sum_t<true, true, 1>(p, s) : phase_sum_t<true, true, 0>(p, s);
-        return parity ? phase_sum_t<true, false, 1>(p, s) : phase_sum_t<true, false, 0>(p, s);
-    }
-    if (half) return parity ? phase_sum_t<false, true, 1>(p, s) : phase_sum_t<false, true, 0>(p, s);
-    return parity ? phase_sum_t<false, false, 1>(p, s) : phase_sum_t<false, false, 0>(p, s);
+hipError_t launch_phase_sum(const KParams& p, bool jet, bool local, hipStream_t s) {
+    if (jet) return local ? phase_sum_t<true, true>(p, s) : phase_sum_t<true, false>(p, s);
+    return local ? phase_sum_t<false, true>(p, s) : phase_sum_t<false, false>(p, s);
 }
 
 hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, int variant, hipStream_t s) {

@@ -1,13 +1,14 @@
 // One timestep of the phase-field LBM on gfx950: three kernels.
 //
-//   k_phase_sum  phi = sum_q g_q                         (src/phaseField.cuh:31-52)
+//   k_phase_sum  phi = sum_q g_q, the g rebuilt from the neighbours' phi and
+//                phase flux (phase_flux.hpp)      (src/phaseField.cuh:31-52)
 //   k_step       fused normals + curvature + CSF force + moments +
-//                regularized collision + f and g streaming
+//                regularized collision + f streaming + phase flux
 //                (src/phaseField.cuh:54-219, src/lbm.cu:33-239)
 //   k_jet_bc     inflow and outflow planes of the jet
 //                (src/BoundaryConditions.cuh:36-169)
 //
-// Streaming is in place with the A-A pattern: one population buffer, two
+// f streams in place with the A-A pattern: one population buffer, two
 // alternating step kinds.  With S[slot][cell]:
 //   even step  in_q  = S[q][x]              out_q -> S[opp q][x]
 //   odd  step  in_q  = S[opp q][x - c_q]    out_q -> S[q][x + c_q]
@@ -18,7 +19,7 @@
 // for the droplet.  The jet's z = 0 and z = nz-1 ghost planes carry the
 // populations the boundary kernel supplies for the next step.
 //
-// k_step marches a 64 x 4 column of cells through z.  A 5-plane ring of the
+// k_step marches a 128 x 4 column of cells through z.  A 5-plane ring of the
 // phi tile with a 2-cell halo and a 3-plane ring of normals with a 1-cell
 // halo live in LDS, so phi is read from HBM about once per cell and the
 // normals never leave the chip (the reference stores normx/y/z, ind and the
@@ -29,6 +30,7 @@
 
 #include "kparams.hpp"
 #include "lattice.hpp"
+#include "phase_flux.hpp"
 
 namespace pf {
 
@@ -116,14 +118,6 @@ template <bool HALF>
 __device__ __forceinline__ void stf(const KParams& p, int slot, Loc l, float v) {
     Pop<HALF>::st(p.f, p.plane_f, p.elems_f, (uint32_t)p.nx, slot, l, v);
 }
-template <bool HALF>
-__device__ __forceinline__ float ldg(const KParams& p, int slot, Loc l) {
-    return Pop<HALF>::ld(p.g, p.plane_g, p.elems_g, (uint32_t)p.nx, slot, l);
-}
-template <bool HALF>
-__device__ __forceinline__ void stg(const KParams& p, int slot, Loc l, float v) {
-    Pop<HALF>::st(p.g, p.plane_g, p.elems_g, (uint32_t)p.nx, slot, l, v);
-}
 
 // Reciprocal: IEEE division by default; the hardware reciprocal (1 ulp)
 // when PF_FAST_RCP is defined.
@@ -136,10 +130,10 @@ __device__ __forceinline__ float recip(float x) {
 }
 
 // Neighbours of one cell (ghost-shell periodicity applied): x, y*pitch of
-// each population array and z for offsets -1, 0, +1 along each axis.
+// the population array and z for offsets -1, 0, +1 along each axis.
 struct Nbr {
     uint32_t xw[3];
-    uint32_t yf[3], yg[3];
+    uint32_t yf[3];
     uint32_t zz[3];
 };
 
@@ -159,48 +153,42 @@ __device__ __forceinline__ Nbr make_nbr(const KParams& p, int x, int y, int z) {
     n.xw[2] = (uint32_t)xp;
     const int yy[3] = {ym, y, yp};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        n.yf[a] = p.pitch_f * (uint32_t)yy[a];
-        n.yg[a] = p.pitch_g * (uint32_t)yy[a];
-    }
+    for (int a = 0; a < 3; ++a) n.yf[a] = p.pitch_f * (uint32_t)yy[a];
     n.zz[0] = (uint32_t)zm;
     n.zz[1] = (uint32_t)z;
     n.zz[2] = (uint32_t)zp;
     return n;
 }
 
-// The cell itself and its neighbour x + SIGN*c_q in the f and g arrays.
+// The cell itself and its neighbour x + SIGN*c_q in the population array.
 __device__ __forceinline__ Loc self_f(const Nbr& n) { return {n.zz[1], n.xw[1] + n.yf[1]}; }
-__device__ __forceinline__ Loc self_g(const Nbr& n) { return {n.zz[1], n.xw[1] + n.yg[1]}; }
 template <int q, int SIGN>
 __device__ __forceinline__ Loc nb_f(const Nbr& n) {
     return {n.zz[SIGN * kCZ[q] + 1], n.xw[SIGN * kCX[q] + 1] + n.yf[SIGN * kCY[q] + 1]};
 }
-template <int q, int SIGN>
-__device__ __forceinline__ Loc nb_g(const Nbr& n) {
-    return {n.zz[SIGN * kCZ[q] + 1], n.xw[SIGN * kCX[q] + 1] + n.yg[SIGN * kCY[q] + 1]};
-}
 
-// Where the next step of parity PAR reads in_q of the cell with neighbour
-// table n, in the f (G = false) or g (G = true) array.
-template <int q, int PAR, bool G>
+// Where the next step of parity PAR reads f_in_q of the cell with
+// neighbour table n.
+template <int q, int PAR>
 __device__ __forceinline__ void in_slot(const Nbr& n, int& slot, Loc& l) {
     if constexpr (PAR == 0) {
         slot = q;
-        l = G ? self_g(n) : self_f(n);
+        l = self_f(n);
     } else {
         slot = opp(q);
-        l = G ? nb_g<q, -1>(n) : nb_f<q, -1>(n);
+        l = nb_f<q, -1>(n);
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_phase_sum: phi = sum_q g_q over interior cells (src/phaseField.cuh:31-52).
-// Each thread sums kPsZ cells stacked in z so that 7*kPsZ loads are in flight.
+// k_phase_sum: phi = sum_q g_q over interior cells (src/phaseField.cuh:31-52),
+// with the seven g rebuilt from the neighbours' phi and phase flux
+// (arriving_g), written to the second phi buffer.  Each thread sums kPsZ
+// cells stacked in z so that their loads are in flight together.
 // ---------------------------------------------------------------------------
 constexpr int kPsZ = 4;
 
-template <bool JET, bool HALF, int PAR>
+template <bool JET, bool LOCAL>
 __global__ __launch_bounds__(kThreads) void k_phase_sum(KParams p) {
     const int x = blockIdx.x * kTX + (threadIdx.x & (kTX - 1));
     const int y = blockIdx.y * kTY + (int)(threadIdx.x / kTX);
@@ -208,36 +196,15 @@ __global__ __launch_bounds__(kThreads) void k_phase_sum(KParams p) {
     const int z0 = 1 + blockIdx.z * kPsZ;
     float g[kPsZ][7];
 #pragma unroll
-    for (int k = 0; k < kPsZ; ++k) {
-        const int z = min(z0 + k, p.nz - 2);  // clamped duplicates are not stored
-        const Nbr n = make_nbr<!JET>(p, x, y, z);
-        static_for<7>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            int slot;
-            Loc l;
-            in_slot<q, PAR, true>(n, slot, l);
-            g[k][q] = ldg<HALF>(p, slot, l);
-        });
-    }
-    if constexpr (HALF) {
-        // binary16: stored g is relative to w_q * (this cell's phi of the
-        // previous step), which the phi field still holds
-#pragma unroll
-        for (int k = 0; k < kPsZ; ++k) {
-            const int z = min(z0 + k, p.nz - 2);
-            const float po = p.fld[10][(uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z)];
-#pragma unroll
-            for (int q = 0; q < 7; ++q) g[k][q] += wq<7>(q) * po;
-        }
-    }
+    for (int k = 0; k < kPsZ; ++k) arriving_g<JET, LOCAL>(p, x, y, min(z0 + k, p.nz - 2), g[k]);
 #pragma unroll
     for (int k = 0; k < kPsZ; ++k) {
         const int z = z0 + k;
-        if (z > p.nz - 2) break;
+        if (z > p.nz - 2) break;  // clamped duplicates are not stored
         float phi = 0.0f;
 #pragma unroll
         for (int q = 0; q < 7; ++q) phi += g[k][q];
-        p.fld[10][(uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z)] = phi;
+        p.phi_next[(uint32_t)x + (uint32_t)p.nx * ((uint32_t)y + (uint32_t)p.ny * (uint32_t)z)] = phi;
     }
 }
 
@@ -437,32 +404,6 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         }
         if (!active) continue;
         const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
-        // binary16 storage keeps g_q relative to w_q * phi of the cell it
-        // streams to (the reference's periodicity): phi of the six face
-        // neighbours, from the LDS tile unless the neighbour wraps; read here,
-        // with the other LDS reads, and used by the g stores.
-        float phn[6] = {0, 0, 0, 0, 0, 0};  // -x, +x, -y, +y, -z, +z
-        if constexpr (HALF) {
-            const float* pl = s_phi[z % 5];
-            const int o2 = (ty + 2) * kHX + (tx + 2);
-            phn[0] = pl[o2 - 1];
-            phn[1] = pl[o2 + 1];
-            phn[2] = pl[o2 - kHX];
-            phn[3] = pl[o2 + kHX];
-            phn[4] = s_phi[(z + 4) % 5][o2];
-            phn[5] = s_phi[(z + 1) % 5][o2];
-            auto at = [&](int gx, int gy, int gz) {
-                return phi_g[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)gz)];
-            };
-            if (x == 1) phn[0] = at(nx - 2, y, z);
-            if (x == nx - 2) phn[1] = at(1, y, z);
-            if (y == 1) phn[2] = at(x, ny - 2, z);
-            if (y == ny - 2) phn[3] = at(x, 1, z);
-            if constexpr (!JET) {
-                if (z == 1) phn[4] = at(x, y, nz - 2);
-                if (z == nz - 2) phn[5] = at(x, y, 1);
-            }
-        }
         if constexpr (MLP) {
             // learned closure at interface cells whose FD curvature lies in
             // the trained range (evaluated by k_closure for this step)
@@ -476,7 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
 
         const Nbr nb = make_nbr<!JET>(p, x, y, z);
-        const Loc sf = self_f(nb), sg = self_g(nb);
+        const Loc sf = self_f(nb);
 
         // ---- relaxation (src/lbm.cu:158-172)
         float omega;
@@ -654,47 +595,15 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             });
         }
 
-        // ---- phase populations (src/lbm.cu:209-238): g_eq + sharpening flux
-        const float sharp = p.gamma * phi * (1.0f - phi);
-
-        if constexpr (LIT) {
-            static_for<7>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                float go = g_post<q>(phi, ux, uy, uz, sharp, nrx, nry, nrz);
-                if constexpr (HALF) {
-                    constexpr int d = kCX[q] != 0 ? (kCX[q] > 0 ? 1 : 0)
-                                                  : (kCY[q] != 0 ? (kCY[q] > 0 ? 3 : 2) : (kCZ[q] > 0 ? 5 : 4));
-                    go -= wq<7>(q) * (q == 0 ? phi : phn[d]);
-                }
-                if constexpr (PAR == 0) {
-                    stg<HALF>(p, opp(q), sg, go);
-                } else {
-                    stg<HALF>(p, q, nb_g<q, 1>(nb), go);
-                }
-            });
-        } else {
-            const float ge = 0.125f * phi;
+        // ---- phase populations (src/lbm.cu:209-238): g_eq + sharpening flux,
+        // kept as the cell's phase flux h (phase_flux.hpp), from which the
+        // next phase sum forms g_0 = phi/4 and g_(+-k) = phi/8 +- h_k
+        {
+            const float sharp = p.gamma * phi * (1.0f - phi);
             const float p4 = 4.0f * phi;
-            const float go_[3] = {0.125f * (p4 * ux + sharp * nrx), 0.125f * (p4 * uy + sharp * nry),
-                                  0.125f * (p4 * uz + sharp * nrz)};
-            float g0 = 0.25f * phi;
-            if constexpr (HALF) g0 -= 0.25f * phi;  // rest population: destination is the cell itself
-            stg<HALF>(p, 0, sg, g0);
-            static_for<3>([&](auto ac) {
-                constexpr int ax = decltype(ac)::value, a = 2 * ax + 1, b = 2 * ax + 2;
-                float gp = ge + go_[ax], gm = ge - go_[ax];
-                if constexpr (HALF) {  // direction a is +axis, b is -axis
-                    gp -= 0.125f * phn[2 * ax + 1];
-                    gm -= 0.125f * phn[2 * ax];
-                }
-                if constexpr (PAR == 0) {
-                    stg<HALF>(p, b, sg, gp);
-                    stg<HALF>(p, a, sg, gm);
-                } else {
-                    stg<HALF>(p, a, nb_g<a, 1>(nb), gp);
-                    stg<HALF>(p, b, nb_g<a, -1>(nb), gm);
-                }
-            });
+            __builtin_nontemporal_store(0.125f * (p4 * ux + sharp * nrx), p.h[0] + c);
+            __builtin_nontemporal_store(0.125f * (p4 * uy + sharp * nry), p.h[1] + c);
+            __builtin_nontemporal_store(0.125f * (p4 * uz + sharp * nrz), p.h[2] + c);
         }
 
         // ---- macroscopic fields for output and for the jet boundary planes
@@ -804,7 +713,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                 }
                 int slot;
                 Loc l;
-                in_slot<q, NPAR, false>(nb, slot, l);
+                in_slot<q, NPAR>(nb, slot, l);
                 stf<HALF>(p, slot, l, val);
             }
         });
@@ -813,7 +722,8 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
             float ux, uy, uz;
             nozzle_u(p, x, y, ux, uy, uz);
             g5 = wq<7>(5) * 1.0f * (1.0f + kAs2g * uz);
-            F[10][pi] = 1.0f;  // ghost phi at z = 0
+            F[10][pi] = 1.0f;  // ghost phi at z = 0, in both phi buffers
+            p.phi_next[pi] = 1.0f;
             if (p.save) {
                 F[0][pi] = 1.0f;
                 F[1][pi] = ux;
@@ -821,10 +731,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                 F[3][pi] = uz;
             }
         }
-        int slot;
-        Loc l;
-        in_slot<5, NPAR, true>(nb, slot, l);
-        stg<HALF>(p, slot, l, HALF ? g5 - wq<7>(5) * F[10][pi + plane] : g5);
+        p.gbc[pi] = g5;  // off the nozzle: the frozen wall equilibrium of phi = 0
     }
 
     // ---------------- outflow, target cell (x, y, nz-2) ----------------
@@ -860,20 +767,17 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                                   0.5f * force_of<Q, q>(cu, ux, uy, uz, tfx, tfy, tfz);
                 int slot;
                 Loc l;
-                in_slot<q, NPAR, false>(nb, slot, l);
+                in_slot<q, NPAR>(nb, slot, l);
                 stf<HALF>(p, slot, l, val);
             }
         });
         const float phi_b = phi_g[c];
         const float uz_b = b[3 * plane + pi];
-        const float g6 = wq<7>(6) * phi_b * (1.0f - kAs2g * uz_b);
-        int slot;
-        Loc l;
-        in_slot<6, NPAR, true>(nb, slot, l);
-        stg<HALF>(p, slot, l, HALF ? g6 - wq<7>(6) * phi_b : g6);
-        // zero-gradient ghost plane z = nz-1
+        p.gbc[plane + pi] = wq<7>(6) * phi_b * (1.0f - kAs2g * uz_b);
+        // zero-gradient ghost plane z = nz-1, in both phi buffers
         const uint32_t cg = c + plane;
         F[10][cg] = phi_b;
+        p.phi_next[cg] = phi_b;
         if (p.save) {
             F[0][cg] = b[0 * plane + pi];
             F[1][cg] = b[1 * plane + pi];
@@ -886,10 +790,10 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
 // ---------------------------------------------------------------------------
 // launchers (instantiated per velocity set in step_q19.hip / step_q27.hip)
 // ---------------------------------------------------------------------------
-template <bool JET, bool HALF, int PAR>
+template <bool JET, bool LOCAL>
 static hipError_t phase_sum_t(const KParams& p, hipStream_t s) {
     dim3 grid((p.nx + kTX - 1) / kTX, (p.ny + kTY - 1) / kTY, (p.nz - 2 + kPsZ - 1) / kPsZ);
-    hipLaunchKernelGGL((k_phase_sum<JET, HALF, PAR>), grid, dim3(kThreads), 0, s, p);
+    hipLaunchKernelGGL((k_phase_sum<JET, LOCAL>), grid, dim3(kThreads), 0, s, p);
     return hipGetLastError();
 }
 

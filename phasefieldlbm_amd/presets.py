@@ -68,12 +68,22 @@ WORKLOADS = {
 
 
 def bytes_per_update(cfg: Config) -> int:
-    """Algorithmic HBM bytes of one cell update: read and write every f and g
-    population once, write and read phi once."""
+    """Algorithmic HBM bytes of one cell update in the reference's scheme:
+    read and write every f and g population once, write and read phi once
+    (f and g in the storage type)."""
     s = 2 if cfg.storage == STORE_FP16 else 4
     return 2 * cfg.velocity_set * s + 2 * 7 * s + 8
 
 
+def bytes_moved_per_update(cfg: Config) -> int:
+    """Algorithmic HBM bytes of one cell update as this library stores the
+    phase field: read and write every f population once; the timestep reads
+    phi and writes the 3-component FP32 phase flux, the phase sum reads phi
+    and the flux and writes phi (2 Q s + 4 + 12 + 4 + 12 + 4)."""
+    s = 2 if cfg.storage == STORE_FP16 else 4
+    return 2 * cfg.velocity_set * s + 36
+
+
 __all__ = ["reference_jet", "reference_droplet", "static_droplet", "round_jet", "jet_d3q19_128x128x512",
-           "jet_d3q27_256x256x1024", "jet_d3q27_128x128x512", "WORKLOADS", "bytes_per_update",
+           "jet_d3q27_256x256x1024", "jet_d3q27_128x128x512", "WORKLOADS", "bytes_per_update", "bytes_moved_per_update",
            "STORE_FP32", "STORE_FP16"]

@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
     const int xs[3] = {xm, x, xp}, ys[3] = {ym, y, yp}, zs[3] = {zm, z, zp};
     if (which == 1) {
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
-        if (interior) arriving_g<JET, LOCAL>(p, x, y, z, g);
+        if (interior) arriving_g<JET, LOCAL, HALF>(p, x, y, z, g);
         for (int q = 0; q < 7; ++q) out[(size_t)q * p.N + i] = g[q];
         return;
     }
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(256) void k_diagnostics(KParams p, double* partials
     block_reduce<6, true>(v, partials + (size_t)blockIdx.x * 6);
 }
 
-template <bool JET, bool LOCAL>
+template <bool JET, bool LOCAL, bool HALF>
 __global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials) {
     double v[1] = {0.0};
     const int nx = p.nx, ny = p.ny, nz = p.nz;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void k_phase_mass(KParams p, double* partials)
         const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
         if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
         float g[7];
-        arriving_g<JET, LOCAL>(p, x, y, z, g);
+        arriving_g<JET, LOCAL, HALF>(p, x, y, z, g);
         double phi = 0.0;
         for (int q = 0; q < 7; ++q) phi += (double)g[q];
         v[0] += phi;
@@ -328,15 +328,17 @@ hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, h
     return hipGetLastError();
 }
 
-template <bool JET>
+template <bool JET, bool HALF>
 static hipError_t mass_t(const KParams& p, bool local, double* partials, int nblocks, hipStream_t s) {
-    if (local) hipLaunchKernelGGL((k_phase_mass<JET, true>), dim3(nblocks), dim3(256), 0, s, p, partials);
-    else hipLaunchKernelGGL((k_phase_mass<JET, false>), dim3(nblocks), dim3(256), 0, s, p, partials);
+    if (local) hipLaunchKernelGGL((k_phase_mass<JET, true, HALF>), dim3(nblocks), dim3(256), 0, s, p, partials);
+    else hipLaunchKernelGGL((k_phase_mass<JET, false, HALF>), dim3(nblocks), dim3(256), 0, s, p, partials);
     return hipGetLastError();
 }
 
-hipError_t launch_phase_mass(const KParams& p, bool jet, bool local, double* partials, int nblocks, hipStream_t s) {
-    return jet ? mass_t<true>(p, local, partials, nblocks, s) : mass_t<false>(p, local, partials, nblocks, s);
+hipError_t launch_phase_mass(const KParams& p, bool jet, bool local, bool half, double* partials, int nblocks,
+                             hipStream_t s) {
+    if (jet) return half ? mass_t<true, true>(p, local, partials, nblocks, s) : mass_t<true, false>(p, local, partials, nblocks, s);
+    return half ? mass_t<false, true>(p, local, partials, nblocks, s) : mass_t<false, false>(p, local, partials, nblocks, s);
 }
 
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s) {

@@ -12,7 +12,8 @@
 //         src/functions/deviceFunctions.cuh:33-48, made timings depend on
 //         where the driver placed the physical pages).  Addressed in place
 //         with the A-A pattern (see the step kernels).
-//   h   : 3 FP32 arrays [N], the phase flux of every cell,
+//   h   : 3 arrays [N] (FP32, binary16 in the binary16 form), the phase
+//         flux of every cell,
 //         h = (4 phi u + gamma phi (1 - phi) n) / 8.  The D3Q7 phase
 //         populations have no non-equilibrium part (D3Q7.cuh:139-143), so the
 //         post-collision g of a cell is fixed by its phi and h:
@@ -66,7 +67,7 @@ constexpr int kMlpHidden = 64;
 
 struct KParams {
     void* f;
-    float* h[3];
+    void* h[3];  // float, or _Float16 in the binary16 form
     float* fld[kNumFields];
     float* phi_next;
     float* bcp;
@@ -102,7 +103,7 @@ struct KParams {
 // Launchers; all return hipError_t of the launch.
 // local: the first step after the populations were set, when each cell's
 // own equilibrium g (not its neighbours' post-collision g) is summed
-hipError_t launch_phase_sum(const KParams& p, bool jet, bool local, hipStream_t s);
+hipError_t launch_phase_sum(const KParams& p, bool jet, bool local, bool half, hipStream_t s);
 // variant: 0 pair-symmetric collision, 1 literal per-direction evaluation,
 // 2 pair-symmetric with the learned curvature closure
 hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, int variant, hipStream_t s);
@@ -113,7 +114,8 @@ hipError_t launch_init_distros(const KParams& p, int Q, bool half, hipStream_t s
 hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, bool local, int which,
                          float* out, hipStream_t s);
 hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, hipStream_t s);
-hipError_t launch_phase_mass(const KParams& p, bool jet, bool local, double* partials, int nblocks, hipStream_t s);
+hipError_t launch_phase_mass(const KParams& p, bool jet, bool local, bool half, double* partials, int nblocks,
+                             hipStream_t s);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
 

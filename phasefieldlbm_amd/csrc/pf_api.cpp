@@ -29,7 +29,7 @@ struct pf_ctx {
     int Q = 19;
     bool jet = true, half = false;
     void* f = nullptr;
-    float* h = nullptr;  // phase flux, 3 x N
+    void* h = nullptr;  // phase flux, 3 x N (FP32, or binary16 in the binary16 form)
     float* fld[pf::kNumFields] = {};
     float* phi_next = nullptr;  // second phi buffer, swapped with fld[PF_FIELD_PHI] every step
     float* bcp = nullptr;
@@ -298,7 +298,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
         c->bytes += bytes;
         return true;
     };
-    bool ok = alloc(&c->f, c->f_elems * pop_bytes(c)) && alloc((void**)&c->h, 3 * N * sizeof(float));
+    bool ok = alloc(&c->f, c->f_elems * pop_bytes(c)) && alloc(&c->h, 3 * N * pop_bytes(c));
     for (int i = 0; ok && i < pf::kNumFields; ++i) ok = alloc((void**)&c->fld[i], N * sizeof(float));
     ok = ok && alloc((void**)&c->phi_next, N * sizeof(float));
     const size_t plane = (size_t)g.nx * g.ny;
@@ -322,7 +322,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
 
     pf::KParams& k = c->kp;
     k.f = c->f;
-    for (int i = 0; i < 3; ++i) k.h[i] = c->h + (size_t)i * N;
+    for (int i = 0; i < 3; ++i) k.h[i] = static_cast<char*>(c->h) + (size_t)i * N * pop_bytes(c);
     for (int i = 0; i < pf::kNumFields; ++i) k.fld[i] = c->fld[i];
     k.phi_next = c->phi_next;
     k.bcp = c->bcp;
@@ -374,7 +374,7 @@ int pf_init_distros(pf_ctx* c) {
     const size_t plane = (size_t)c->cfg.nx * c->cfg.ny;
     PF_HIP(c, hipMemsetAsync(c->bcp, 0, 2 * pf::kPlaneVals * plane * sizeof(float), c->stream));
     PF_HIP(c, hipMemsetAsync(c->gbc, 0, 2 * plane * sizeof(float), c->stream));
-    PF_HIP(c, hipMemsetAsync(c->h, 0, 3 * c->N * sizeof(float), c->stream));
+    PF_HIP(c, hipMemsetAsync(c->h, 0, 3 * c->N * pop_bytes(c), c->stream));
     // the ghost shell of phi is read from whichever buffer is current
     PF_HIP(c, hipMemcpyAsync(c->phi_next, c->fld[PF_FIELD_PHI], c->N * sizeof(float), hipMemcpyDeviceToDevice,
                              c->stream));
@@ -456,7 +456,7 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
         c->kp.save = (avg || ((flags & PF_STEP_SAVE_LAST) && s + 1 == nsteps)) ? 1 : 0;
         const int variant = c->cfg.closure == PF_CLOSURE_MLP ? 2 : c->cfg.kernel;
         if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
-        PF_HIP(c, pf::launch_phase_sum(c->kp, c->jet, c->fresh, c->stream));
+        PF_HIP(c, pf::launch_phase_sum(c->kp, c->jet, c->fresh, c->half, c->stream));
         swap_phi(c);
         c->fresh = false;
         if (variant == 2) PF_HIP(c, pf::launch_closure(c->kp, c->stream));
@@ -541,7 +541,7 @@ int pf_phase_mass(pf_ctx* c, double* mass) {
     if (!c || !mass) return PF_ERR_CONFIG;
     if (!c->inited) return fail(c, PF_ERR_STATE, "phase mass before pf_init");
     PF_HIP(c, hipSetDevice(c->dev));
-    PF_HIP(c, pf::launch_phase_mass(c->kp, c->jet, c->fresh, c->partials, c->nred, c->stream));
+    PF_HIP(c, pf::launch_phase_mass(c->kp, c->jet, c->fresh, c->half, c->partials, c->nred, c->stream));
     PF_HIP(c, pf::launch_reduce_partials(c->partials, c->nred, 1, c->dout, c->stream));
     PF_HIP(c, hipMemcpyAsync(mass, c->dout, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PF_HIP(c, hipStreamSynchronize(c->stream));

@@ -9,13 +9,17 @@
 // (D3Q7.cuh:129-153 with w = 1/4, 1/8 and c_s^2 = 1/4).  The timestep stores h
 // at the cell (3 values) and the phase kernel forms the g arriving at x from
 // phi and h of its six neighbours, in the same FP32 operations the
-// post-collision g would have been formed with, so phi is unchanged.
+// post-collision g would have been formed with, so phi is unchanged.  In the
+// binary16 form h is stored in binary16; the h of each neighbour enters once
+// with + and once with - across the grid, so sum phi stays conserved.
 //
 // Jet planes: the g_5 entering z = 1 and the g_6 entering z = nz-2 are set by
 // the boundary kernel (src/BoundaryConditions.cuh:104,168) and kept in gbc.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "kparams.hpp"
 #include "lattice.hpp"
@@ -26,8 +30,12 @@ namespace pf {
 // (rest, +x, -x, +y, -y, +z, -z).  LOCAL: the first step after the
 // populations were set, when every cell holds its own equilibrium
 // (setDistros, src/initialConditions.cu:101-138) and sums that.
-template <bool JET, bool LOCAL>
+template <bool JET, bool LOCAL, bool HALF>
 __device__ __forceinline__ void arriving_g(const KParams& p, int x, int y, int z, float (&g)[7]) {
+    using HT = typename std::conditional<HALF, _Float16, float>::type;
+    const HT* __restrict__ h0 = static_cast<const HT*>(p.h[0]);
+    const HT* __restrict__ h1 = static_cast<const HT*>(p.h[1]);
+    const HT* __restrict__ h2 = static_cast<const HT*>(p.h[2]);
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)ny;
     const uint32_t c = (uint32_t)x + row * (uint32_t)y + plane * (uint32_t)z;
@@ -51,17 +59,17 @@ __device__ __forceinline__ void arriving_g(const KParams& p, int x, int y, int z
     const uint32_t cym = c + row * (uint32_t)(ym - y), cyp = c + row * (uint32_t)(yp - y);
     const uint32_t czm = c + plane * (uint32_t)(zm - z), czp = c + plane * (uint32_t)(zp - z);
     g[0] = 0.25f * phi[c];
-    g[1] = 0.125f * phi[cxm] + p.h[0][cxm];
-    g[2] = 0.125f * phi[cxp] - p.h[0][cxp];
-    g[3] = 0.125f * phi[cym] + p.h[1][cym];
-    g[4] = 0.125f * phi[cyp] - p.h[1][cyp];
+    g[1] = 0.125f * phi[cxm] + (float)h0[cxm];
+    g[2] = 0.125f * phi[cxp] - (float)h0[cxp];
+    g[3] = 0.125f * phi[cym] + (float)h1[cym];
+    g[4] = 0.125f * phi[cyp] - (float)h1[cyp];
     if constexpr (JET) {
         const uint32_t pi = (uint32_t)x + row * (uint32_t)y;
-        g[5] = (z == 1) ? p.gbc[pi] : 0.125f * phi[czm] + p.h[2][czm];
-        g[6] = (z == nz - 2) ? p.gbc[plane + pi] : 0.125f * phi[czp] - p.h[2][czp];
+        g[5] = (z == 1) ? p.gbc[pi] : 0.125f * phi[czm] + (float)h2[czm];
+        g[6] = (z == nz - 2) ? p.gbc[plane + pi] : 0.125f * phi[czp] - (float)h2[czp];
     } else {
-        g[5] = 0.125f * phi[czm] + p.h[2][czm];
-        g[6] = 0.125f * phi[czp] - p.h[2][czp];
+        g[5] = 0.125f * phi[czm] + (float)h2[czm];
+        g[6] = 0.125f * phi[czp] - (float)h2[czp];
     }
 }
 

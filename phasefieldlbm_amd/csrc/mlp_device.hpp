@@ -44,10 +44,16 @@ __device__ __forceinline__ float fast_tanh(float u) {
     return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
 }
 
-// GELU, tanh form, and its derivative.
+// GELU, tanh form, and its derivative.  The value is evaluated through the
+// identity 0.5 x (1 + tanh u) = x / (1 + e^(-2u)), u = sqrt(2/pi) (x +
+// 0.044715 x^3): one exponential and one reciprocal.  For large negative x,
+// e^(-2u) overflows to +inf and the result is x * 0, the limit; for large
+// positive x it is x.
 __device__ __forceinline__ float gelu(float x) {
-    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
-    return 0.5f * x * (1.0f + fast_tanh(u));
+    constexpr float k1 = -2.0f * 0.7978845608028654f * 1.4426950408889634f;  // -2 sqrt(2/pi) log2(e)
+    constexpr float k3 = k1 * 0.044715f;
+    const float v = x * (k1 + k3 * (x * x));
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(v));
 }
 __device__ __forceinline__ float gelu_grad(float x) {
     const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
@@ -97,14 +103,18 @@ __device__ __forceinline__ float mlp_eval16(const MlpWeightsLds& s, bf16x8 xh, b
         bf16x8 a;
 #pragma unroll
         for (int r = 0; r < 8; ++r) a[r] = W1[(16 * m + i) * kMlpInPad + 8 * g + r];
-        d1[m] = mfma32(a, xl, mfma32(a, xh, f32x4{0.0f, 0.0f, 0.0f, 0.0f}));
+        // the bias seeds the accumulator (row 16m + 4g + r of this lane)
+        const f32x4 b1 = {s.b[16 * m + 4 * g], s.b[16 * m + 4 * g + 1], s.b[16 * m + 4 * g + 2],
+                          s.b[16 * m + 4 * g + 3]};
+        d1[m] = mfma32(a, xl, mfma32(a, xh, b1));
 #pragma unroll
-        for (int r = 0; r < 4; ++r) d1[m][r] = gelu(d1[m][r] + s.b[16 * m + 4 * g + r]);
+        for (int r = 0; r < 4; ++r) d1[m][r] = gelu(d1[m][r]);
     }
     float y = 0.0f;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int hb = kMlpHidden + 16 * m + 4 * g;
+        f32x4 acc = {s.b[hb], s.b[hb + 1], s.b[hb + 2], s.b[hb + 3]};
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             bf16x8 a, b;
@@ -120,7 +130,7 @@ __device__ __forceinline__ float mlp_eval16(const MlpWeightsLds& s, bf16x8 xh, b
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int h = 16 * m + 4 * g + r;
-            y += (float)w3[h] * gelu(acc[r] + s.b[kMlpHidden + h]);
+            y += (float)w3[h] * gelu(acc[r]);
         }
     }
     y += __shfl_xor(y, 16, 64);

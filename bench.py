@@ -6,7 +6,8 @@ nozzle D=32, u=0.05, Re=5000, We=500, FP32 populations.  Inputs are
 resident in HBM before timing starts.  Also reported: the FP16-storage form
 of the same workload, the same workload with the learned curvature closure
 (its cost in MLUPS), the Laplace pressure error of static droplets
-(R = 8, 12, 16) and the CPU oracle timed on the host on a bounded sample.
+(R = 8, 12, 16), the MLUPS of the D3Q19 128x128x512 jet and the 64^3
+droplet, and the CPU oracle timed on the host on a bounded sample.
 
 MLUPS counts nx*ny*nz cells per step, as the reference does
 (src/main.cu:203-205); the interior-cell rate is reported beside it.
@@ -33,9 +34,9 @@ def log(msg):
 
 def kernel_bytes_per_cell(q, s):
     """Algorithmic bytes of the fused collide-stream kernel per interior cell:
-    read and write Q f populations, write the 3-component FP32 phase flux,
-    read phi."""
-    return 2 * q * s + 12 + 4
+    read and write Q f populations, write the 3-component phase flux (both
+    in the storage type), read phi."""
+    return 2 * q * s + 3 * s + 4
 
 
 def run_form(pf, cfg, steps, warmup, closure_params=None):
@@ -171,10 +172,13 @@ def main():
         wall16, tm16, _ = run_form(pf, c16, args.steps, args.warmup)
         mlups16 = cells * args.steps / wall16 / 1e6
         b16 = presets.bytes_per_update(c16)
+        b16m = presets.bytes_moved_per_update(c16)
         out["fp16_storage"] = {"value": round(mlups16, 1), "unit": "MLUPS",
                                "speedup_vs_fp32": round(mlups16 / mlups, 3),
                                "step_roofline_frac": round(mlups16 * 1e6 * b16 / (HBM_PEAK_GBS * 1e9), 4),
-                               "kernel_ms": round(tm16["step_ms"] / args.steps, 4), "bytes_per_update": b16}
+                               "moved_roofline_frac": round(mlups16 * 1e6 * b16m / (HBM_PEAK_GBS * 1e9), 4),
+                               "kernel_ms": round(tm16["step_ms"] / args.steps, 4), "bytes_per_update": b16,
+                               "bytes_moved_per_update": b16m}
         log(f"[bench] fp16 storage: {mlups16:.0f} MLUPS ({mlups16 / mlups:.2f}x)")
         # Learned closure against the FD curvature on a developed jet (its
         # interface has grown for `closure_warmup` steps), same step count.
@@ -195,6 +199,15 @@ def main():
         log(f"[bench] learned closure after {cw} steps: {mlupsm:.0f} MLUPS against {mlupsf:.0f} with FD "
             f"(cost {100 * (1 - mlupsm / mlupsf):.1f} %)")
         out["laplace"] = [laplace(pf, presets, R) for R in (8, 12, 16)]
+        # the other configurations of the workload set, FP32, for reference
+        others = {}
+        for name in ("jet_d3q19_128x128x512", "droplet_d3q19_64"):
+            make, _ = presets.WORKLOADS[name]
+            c = make()
+            w, _, _ = run_form(pf, c, args.steps, args.warmup)
+            others[name] = round(c.nx * c.ny * c.nz * args.steps / w / 1e6, 1)
+        out["other_workloads_mlups"] = others
+        log(f"[bench] other workloads: {others}")
         log(f"[bench] laplace: {[round(r['rel_err'], 4) for r in out['laplace']]}")
 
     if not args.no_cpu:

@@ -78,10 +78,11 @@ def bytes_per_update(cfg: Config) -> int:
 def bytes_moved_per_update(cfg: Config) -> int:
     """Algorithmic HBM bytes of one cell update as this library stores the
     phase field: read and write every f population once; the timestep reads
-    phi and writes the 3-component FP32 phase flux, the phase sum reads phi
-    and the flux and writes phi (2 Q s + 4 + 12 + 4 + 12 + 4)."""
+    phi and writes the 3-component phase flux, the phase sum reads phi and
+    the flux and writes phi (2 Q s + 3 s + 4 + 3 s + 8; f and the flux in
+    the storage type, phi in FP32)."""
     s = 2 if cfg.storage == STORE_FP16 else 4
-    return 2 * cfg.velocity_set * s + 36
+    return 2 * cfg.velocity_set * s + 6 * s + 12
 
 
 __all__ = ["reference_jet", "reference_droplet", "static_droplet", "round_jet", "jet_d3q19_128x128x512",

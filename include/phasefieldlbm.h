@@ -33,9 +33,10 @@ enum pf_velocity_set { PF_D3Q19 = 19, PF_D3Q27 = 27 };
 enum pf_flow_case { PF_JET = 0, PF_DROPLET = 1 };
 
 /* Population storage.  FP32 is the plain path.  FP16 stores f - w*rho0 and
- * g_q - w_q*phi(destination cell) in binary16 (round to nearest even) with
- * FP32 arithmetic, the role of ENABLE_FP16 (src/cuda/precision.cuh:29-42),
- * which in the reference stores only f in FP16. */
+ * the phase flux (the 3-vector from which the D3Q7 populations are rebuilt)
+ * in binary16 (round to nearest even) with FP32 arithmetic, the role of
+ * ENABLE_FP16 (src/cuda/precision.cuh:29-42), which in the reference stores
+ * f in FP16 and g in FP32. */
 enum pf_storage { PF_STORE_FP32 = 0, PF_STORE_FP16 = 1 };
 
 /* Inflow perturbation.  GAUSSIAN_XY is the reference's Box-Muller noise with
@@ -130,7 +131,7 @@ typedef struct pf_config {
     /* phase field */
     float interface_width;
     float tau_g;           /* only sets the sharpening gamma, as in the reference */
-    float phi0;            /* unused: FP16 storage keeps g_q relative to w_q*phi of the cell it streams to */
+    float phi0;            /* unused: the D3Q7 populations are not stored (see pf_get_populations) */
     /* execution */
     int32_t device;        /* HIP device index; <0 reads GPU_INDEX like setDeviceFromEnv */
     int32_t zchunk;        /* z cells marched per workgroup; 0 = automatic */
@@ -181,7 +182,8 @@ int pf_get_field(pf_ctx *ctx, int32_t id, float *host, size_t n);
 
 /* Copy the post-streaming populations f (which=0, Q*N) or g (which=1, 7*N)
  * in the reference's layout f[q*N + idx], unshifted for g and shifted by -w
- * for f (the reference's stored value), as FP32. */
+ * for f (the reference's stored value), as FP32.  g is rebuilt from phi and
+ * the stored phase flux: the populations the next phase sum adds. */
 int pf_get_populations(pf_ctx *ctx, int32_t which, float *host, size_t n);
 
 /* FP64 reductions over interior cells of the last saved fields:

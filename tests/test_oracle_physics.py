@@ -102,3 +102,47 @@ def test_laplace_law_fp64():
     dp = (inside - outside) / 3.0
     err = abs(dp - 2 * sigma / R) / (2 * sigma / R)
     assert err < 0.25, (dp, 2 * sigma / R)
+
+
+@pytest.mark.parametrize("case", ["droplet", "jet"])
+def test_phase_flux_identity_fp64(case):
+    """The D3Q7 populations carry no non-equilibrium part
+    (src/velocitySet/D3Q7.cuh:139-143), so the g that reaches x from its
+    neighbour s = x - c_q is phi(s)/8 + c_q.h(s), with
+    h = (4 phi u + gamma phi (1 - phi) n) / 8 of that neighbour, and
+    g_0 = phi(x)/4.  The library stores h instead of the seven g and rebuilds
+    them this way; checked here on the FP64 oracle's own populations, which
+    store and stream g as the reference does."""
+    if case == "droplet":
+        cfg = presets.static_droplet(n=20, radius=5.0, velocity_set=D3Q19)
+        steps = 7
+    else:
+        cfg = presets.round_jet(16, 16, 32, 6, D3Q19, noise=NOISE_UNIFORM_Z)
+        steps = 9
+    o = oracle.Oracle(cfg, 64).init().step(steps)
+    phi = o.field("phi")
+    u = [o.field(k) for k in ("ux", "uy", "uz")]
+    n = [o.field(k) for k in ("normx", "normy", "normz")]
+    # gamma = 4 (tau_g - 1/2) / W, from the float32 values of pf_config
+    tau_g, width = float(np.float32(cfg.tau_g)), float(np.float32(cfg.interface_width))
+    gamma = (4.0 * (0.25 * (tau_g - 0.5)) / width) / 0.25
+    h = [(4.0 * phi * u[a] + gamma * phi * (1.0 - phi) * n[a]) / 8.0 for a in range(3)]
+    g = o.populations("g")
+    o.close()
+    nz, ny, nx = phi.shape
+    z0, z1 = (2, nz - 2) if case == "jet" else (1, nz - 1)  # jet: z = 1, nz-2 take boundary g
+    z, y, x = np.meshgrid(np.arange(z0, z1), np.arange(1, ny - 1), np.arange(1, nx - 1), indexing="ij")
+
+    def wrap(i, m):  # ghost-shell periodicity over interior cells
+        return np.where(i < 1, m - 2, np.where(i > m - 2, 1, i))
+
+    assert np.abs(g[0][z, y, x] - phi[z, y, x] / 4.0).max() < 1e-15
+    dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]  # D3Q7 order q = 1..6
+    for q, c in enumerate(dirs, start=1):
+        sx, sy, sz = wrap(x - c[0], nx), wrap(y - c[1], ny), wrap(z - c[2], nz)
+        a = int(np.flatnonzero(c)[0])
+        want = phi[sz, sy, sx] / 8.0 + c[a] * h[a][sz, sy, sx]
+        assert np.abs(g[q][z, y, x] - want).max() < 1e-15, q
+    # and a planted error is seen: the flux of the cell itself instead of the source
+    bad = phi[z, y, x] / 8.0 + h[0][z, y, x]
+    assert np.abs(g[1][z, y, x] - bad).max() > 1e-6

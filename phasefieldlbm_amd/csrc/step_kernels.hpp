@@ -378,10 +378,25 @@ __device__ __forceinline__ void pair_out(const Coll& k, float ux, float uy, floa
     out_m = even - odd;
 }
 
+// LDS of k_step.  Full: a 5-plane phi ring and 3 planes of (n, |grad phi|)
+// with a 1-cell halo, 58.9 KB, 2 workgroups per CU.  Slim: a 4-plane phi ring,
+// 3 planes of n and 2 planes of |grad phi| at the tile cells only (it is read
+// at the centre only), 49.1 KB, 3 workgroups per CU.
+#ifndef PF_SLIM_LDS_FP32
+#define PF_SLIM_LDS_FP32 0
+#endif
+#ifndef PF_SLIM_LDS_FP16
+#define PF_SLIM_LDS_FP16 1
+#endif
+
 template <int Q, bool JET, bool HALF, int PAR, bool MLP, bool LIT, bool SAVE>
 __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
-    __shared__ float s_phi[5][kHY * kHX];
-    __shared__ float s_n[3][4][kNY1 * kNX1];
+    constexpr bool SLIM = HALF ? (PF_SLIM_LDS_FP16 != 0) : (PF_SLIM_LDS_FP32 != 0);
+    constexpr int kRing = SLIM ? 4 : 5;  // phi planes
+    constexpr int kNC = SLIM ? 3 : 4;    // components per normal plane
+    __shared__ float s_phi[kRing][kHY * kHX];
+    __shared__ float s_n[3][kNC][kNY1 * kNX1];
+    __shared__ float s_ind[SLIM ? 2 : 1][SLIM ? kThreads : 1];
     constexpr int NP = (Q - 1) / 2;
 
     const int tid = threadIdx.x;
@@ -398,7 +413,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     const float* __restrict__ phi_g = p.fld[10];
 
     auto load_phi = [&](int pz) {
-        float* dst = s_phi[(pz + 5) % 5];
+        float* dst = s_phi[(pz + kRing) % kRing];
         for (int i = tid; i < kHX * kHY; i += kThreads) {
             const int lx = i % kHX, ly = i / kHX;
             const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
@@ -414,9 +429,9 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     // never writes.
     auto comp_normals = [&](int pz) {
         const int r = pz % 3;
-        const float* pm = s_phi[(pz + 4) % 5];
-        const float* p0 = s_phi[pz % 5];
-        const float* pp = s_phi[(pz + 1) % 5];
+        const float* pm = s_phi[(pz + kRing - 1) % kRing];
+        const float* p0 = s_phi[pz % kRing];
+        const float* pp = s_phi[(pz + 1) % kRing];
         for (int i = tid; i < kNX1 * kNY1; i += kThreads) {
             const int lx = i % kNX1, ly = i / kNX1;
             const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
@@ -441,7 +456,11 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             s_n[r][0][i] = nrx;
             s_n[r][1][i] = nry;
             s_n[r][2][i] = nrz;
-            s_n[r][3][i] = ind;
+            if constexpr (SLIM) {
+                if (lx >= 1 && lx <= kTX && ly >= 1 && ly <= kTY) s_ind[pz & 1][(ly - 1) * kTX + (lx - 1)] = ind;
+            } else {
+                s_n[r][3][i] = ind;
+            }
         }
     };
 
@@ -450,6 +469,8 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     __syncthreads();
     comp_normals(zs - 1);
     comp_normals(zs);
+    // the 4-plane ring reuses the slot of phi(zs - 2) for the first load below
+    if constexpr (SLIM) __syncthreads();
 
     const int x = x0 + tx, y = y0 + ty;
     const bool active = (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2);
@@ -468,7 +489,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             constexpr int off = kCY[q] * kNX1 + kCX[q];
             return nm[kCZ[q] + 1][comp_ * (kNY1 * kNX1) + o + off];
         };
-        const float phi = s_phi[z % 5][(ty + 2) * kHX + (tx + 2)];
+        const float phi = s_phi[z % kRing][(ty + 2) * kHX + (tx + 2)];
         float curvature = 0.0f;
         if (active) {
             const float scx = iso_sum<Q, 0>([&](auto qc) { return nval(0, qc); });
@@ -485,7 +506,11 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 curvature = p.kappa[c];
         }
 
-        const float ind = s_n[z % 3][3][o];
+        float ind;
+        if constexpr (SLIM)
+            ind = s_ind[z & 1][tid];
+        else
+            ind = s_n[z % 3][3][o];
         const float nrx = s_n[z % 3][0][o], nry = s_n[z % 3][1][o], nrz = s_n[z % 3][2][o];
         const float stCurv = -p.sigma * curvature * ind;
         const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;

@@ -151,13 +151,14 @@ void fill_constants(pf_ctx* c) {
 
 // z cells marched per workgroup.  Measured on MI355X (D3Q27 jet
 // 256x256x1024): FP32 is fastest with ~16k workgroups (z-chunk 8), the
-// binary16 form with ~4k (z-chunk 32); short chunks re-load more halo planes,
-// long ones leave the last wave of workgroups unbalanced.
+// binary16 form (3 workgroups per CU) with ~8k (z-chunk 16); short chunks
+// re-load more halo planes, long ones leave the last wave of workgroups
+// unbalanced.
 int choose_zchunk(const pf_config& g) {
     if (g.zchunk > 0) return g.zchunk;
     const int ntx = (g.nx + pf::kTX - 1) / pf::kTX, nty = (g.ny + pf::kTY - 1) / pf::kTY;
     const long cols = (long)ntx * nty;
-    const long target = g.storage == PF_STORE_FP16 ? 4096 : 16384;
+    const long target = g.storage == PF_STORE_FP16 ? 8192 : 16384;
     long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;

@@ -236,11 +236,13 @@ def test_step_chunking_is_invisible():
             np.testing.assert_array_equal(a.field(k), b.field(k))
 
 
+@pytest.mark.parametrize("storage", [0, STORE_FP16])
 @pytest.mark.parametrize("zchunk", [1, 3, 64])
-def test_zchunk_invariance(zchunk):
-    """The z-marching tile length changes the schedule, not the result."""
-    cfg = presets.round_jet(24, 24, 40, 6, D3Q27, noise=NOISE_UNIFORM_Z)
-    ref = presets.round_jet(24, 24, 40, 6, D3Q27, noise=NOISE_UNIFORM_Z)
+def test_zchunk_invariance(zchunk, storage):
+    """The z-marching tile length changes the schedule, not the result (both
+    LDS layouts: FP32 uses the 5-plane phi ring, binary16 the 4-plane one)."""
+    cfg = presets.round_jet(24, 24, 40, 6, D3Q27, storage=storage, noise=NOISE_UNIFORM_Z)
+    ref = presets.round_jet(24, 24, 40, 6, D3Q27, storage=storage, noise=NOISE_UNIFORM_Z)
     cfg.zchunk = zchunk
     ref.zchunk = 0
     with pf.Simulation(cfg) as a, pf.Simulation(ref) as b:

@@ -5,7 +5,8 @@ D3Q27 regularized hydrodynamics + D3Q7 phase field, 256x256x1024 cells,
 nozzle D=32, u=0.05, Re=5000, We=500, FP32 populations.  Inputs are
 resident in HBM before timing starts.  Also reported: the FP16-storage form
 of the same workload, the same workload with the learned curvature closure
-(its cost in MLUPS), the Laplace pressure error of static droplets
+(its cost in MLUPS, and its Laplace pressure error against the FD curvature
+for R = 6..20), the Laplace pressure error of static droplets
 (R = 8, 12, 16), the MLUPS of the D3Q19 128x128x512 jet and the 64^3
 droplet, and the CPU oracle timed on the host on a bounded sample.
 
@@ -198,6 +199,17 @@ def main():
                                   "umax": diagm["umax"]}
         log(f"[bench] learned closure after {cw} steps: {mlupsm:.0f} MLUPS against {mlupsf:.0f} with FD "
             f"(cost {100 * (1 - mlupsm / mlupsf):.1f} %)")
+        # Laplace pressure jump of static droplets (64^3, 3000 steps, tau_g = 1,
+        # the closure's training regime): FD curvature against the closure
+        rows = closure.laplace(closure.load(), radii=(6, 7, 8, 10, 12, 14, 16, 20))
+        out["learned_closure"]["laplace"] = [
+            {"R": r["R"], "fd_rel_err": round(r["fd"]["rel_err"], 5), "mlp_rel_err": round(r["mlp"]["rel_err"], 5),
+             "fd_spurious_umax": r["fd"]["spurious_umax"], "mlp_spurious_umax": r["mlp"]["spurious_umax"]}
+            for r in rows]
+        out["learned_closure"]["laplace_mean_rel_err"] = {
+            "fd": round(float(np.mean([r["fd"]["rel_err"] for r in rows])), 5),
+            "mlp": round(float(np.mean([r["mlp"]["rel_err"] for r in rows])), 5)}
+        log(f"[bench] closure laplace mean |dP err|: {out['learned_closure']['laplace_mean_rel_err']}")
         out["laplace"] = [laplace(pf, presets, R) for R in (8, 12, 16)]
         # the other configurations of the workload set, FP32, for reference
         others = {}

@@ -91,6 +91,15 @@ def test_literal_kernel(Q, case):
 
 
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+def test_literal_kernel_fp16(Q):
+    """The literal evaluation in the binary16 form (slim LDS layout) against
+    the binary16 oracle."""
+    cfg = presets.round_jet(20, 20, 40, 6, Q, storage=STORE_FP16, noise=NOISE_UNIFORM_Z)
+    cfg.kernel = pf.KERNEL_LITERAL
+    compare(cfg, 60, prec=16, tol=2e-3)
+
+
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
 def test_pair_vs_literal_kernel(Q):
     """Pair-symmetric and literal evaluation agree to FP32 round-off."""
     cfg = presets.round_jet(24, 24, 48, 8, Q, noise=NOISE_UNIFORM_Z)

@@ -98,15 +98,17 @@ def test_mlp_training_fits():
 
 
 @pytest.mark.gpu
-def test_closure_in_timestep():
+@pytest.mark.parametrize("storage", [0, 1])
+def test_closure_in_timestep(storage):
     """k_closure + the closure variant of the fused step: the curvature the
     step applies at interface cells equals the reference network evaluated
-    on the same phi stencils (and their point reflections)."""
+    on the same phi stencils (and their point reflections).  Both storage
+    forms (binary16 uses the step's slim LDS layout)."""
     import phasefieldlbm_amd as pf
     from phasefieldlbm_amd import closure, presets
     rng = np.random.default_rng(5)
     p = (rng.normal(size=6017) * 0.1).astype(np.float32)
-    cfg = presets.static_droplet(n=32, radius=8.0)
+    cfg = presets.static_droplet(n=32, radius=8.0, storage=storage)
     cfg.closure = pf.CLOSURE_MLP
     cfg.mlp_kmin, cfg.mlp_kmax = -100.0, 100.0  # network at every interface cell
     with pf.Simulation(cfg) as sim:

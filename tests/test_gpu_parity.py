@@ -179,17 +179,22 @@ def test_fp16_storage_against_fp16_oracle(Q):
             tol=2e-3)
 
 
-@pytest.mark.parametrize("steps", [4, 5])
+@pytest.mark.parametrize("steps", [0, 1, 4, 5])
 @pytest.mark.parametrize("case", ["droplet", "jet"])
 def test_populations(steps, case):
-    """Post-streaming populations in the reference layout after an even and
-    an odd number of in-place steps."""
+    """Post-streaming populations in the reference layout right after
+    initialisation (g as setDistros leaves them), after the first step, and
+    after an even and an odd number of in-place steps."""
     cfg = (presets.static_droplet(n=16, radius=4.0, velocity_set=D3Q27) if case == "droplet"
            else presets.round_jet(16, 16, 32, 6, D3Q27, noise=NOISE_UNIFORM_Z))
     with pf.Simulation(cfg) as sim:
-        sim.init().step(steps)
+        sim.init()
+        if steps:
+            sim.step(steps)
         f, g = sim.populations("f"), sim.populations("g")
-    o = oracle.Oracle(cfg, 32).init().step(steps)
+    o = oracle.Oracle(cfg, 32).init()
+    if steps:
+        o.step(steps)
     J = (slice(None),) + I
     assert np.abs(f[J] - o.populations("f")[J]).max() / pop_scale(cfg) < 1e-5
     assert np.abs(g[J] - o.populations("g")[J]).max() / 0.25 < 1e-5

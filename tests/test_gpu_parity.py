@@ -90,6 +90,24 @@ def test_literal_kernel(Q, case):
     compare(cfg, 60, fields=pf.FIELDS)
 
 
+@pytest.mark.parametrize("case", ["droplet", "jet"])
+def test_slim_lds_layout_matches_full(case):
+    """After one step from the same initial state both storage forms see the
+    same phi (the first phase sum uses the FP32 fields only), so the phase
+    stencil fields are identical bit for bit although the binary16 form
+    stages them in the slim LDS layout (4-plane phi ring, |grad phi| in its
+    own ring) and FP32 in the full one."""
+    mk = ((lambda st: presets.static_droplet(n=24, radius=6.0, velocity_set=D3Q27, storage=st)) if case == "droplet"
+          else (lambda st: presets.round_jet(20, 20, 40, 6, D3Q27, storage=st, noise=NOISE_UNIFORM_Z)))
+    out = {}
+    for st in (0, STORE_FP16):
+        with pf.Simulation(mk(st)) as sim:
+            sim.init().step(1, save=True)
+            out[st] = {k: sim.field(k) for k in ("phi", "normx", "normy", "normz", "ind", "fsx", "fsy", "fsz")}
+    for k in out[0]:
+        np.testing.assert_array_equal(out[0][k], out[STORE_FP16][k], err_msg=k)
+
+
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
 def test_literal_kernel_fp16(Q):
     """The literal evaluation in the binary16 form (slim LDS layout) against

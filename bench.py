@@ -7,8 +7,9 @@ resident in HBM before timing starts.  Also reported: the FP16-storage form
 of the same workload, the same workload with the learned curvature closure
 (its cost in MLUPS, and its Laplace pressure error against the FD curvature
 for R = 6..20), the Laplace pressure error of static droplets
-(R = 8, 12, 16), the MLUPS of the D3Q19 128x128x512 jet and the 64^3
-droplet, and the CPU oracle timed on the host on a bounded sample.
+(R = 8, 12, 16), the MLUPS of the D3Q19 128x128x512 jet, the 64^3
+droplet and the reference's own default 64x64x128 jet, and the CPU oracle
+timed on the host on a bounded sample.
 
 MLUPS counts nx*ny*nz cells per step, as the reference does
 (src/main.cu:203-205); the interior-cell rate is reported beside it.
@@ -225,6 +226,18 @@ def main():
             others[name] = round(c.nx * c.ny * c.nz * args.steps / w / 1e6, 1)
         out["other_workloads_mlups"] = others
         log(f"[bench] other workloads: {others}")
+        # the reference's own default jet (64x64x128, D = 10, Gaussian inflow
+        # noise), at which its README quotes 470 (D3Q19) and 410 (D3Q27)
+        # MLUPS on an RTX 3050, probably with binary16 f; context only
+        ref_default = {}
+        for vs in (pf.D3Q19, pf.D3Q27):
+            for st, tag in ((pf.STORE_FP32, "fp32"), (pf.STORE_FP16, "fp16")):
+                c = presets.reference_jet(vs)
+                c.storage = st
+                w, _, _ = run_form(pf, c, args.steps, args.warmup)
+                ref_default[f"d3q{vs}_{tag}"] = round(c.nx * c.ny * c.nz * args.steps / w / 1e6, 1)
+        out["reference_default_jet_mlups"] = ref_default
+        log(f"[bench] reference default jet 64x64x128: {ref_default}")
         log(f"[bench] laplace: {[round(r['rel_err'], 4) for r in out['laplace']]}")
 
     if not args.no_cpu:

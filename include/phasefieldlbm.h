@@ -140,6 +140,7 @@ typedef struct pf_config {
     int32_t slot_pad;      /* elements of padding per population row (beyond Q*nx); <0 = automatic (0) */
     float mlp_kmin, mlp_kmax; /* MLP applies where the FD curvature lies in [kmin, kmax] */
     int32_t derived;       /* pf_derived bit mask; 0 = none (the reference's default build) */
+    int32_t tile_x;        /* timestep tile width: 0 = automatic (64 when nx <= 64, else 128), 64 or 128 */
 } pf_config;
 
 typedef struct pf_ctx pf_ctx;

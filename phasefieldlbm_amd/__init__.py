@@ -83,6 +83,7 @@ class Config:
     mlp_kmin: float = -0.6
     mlp_kmax: float = -0.06
     derived: int = 0  # DERIVED_AVERAGE | DERIVED_REYNOLDS | DERIVED_VORTICITY
+    tile_x: int = 0  # timestep tile width: 0 = automatic, 64 or 128
 
     @classmethod
     def reference(cls, flow_case: int, velocity_set: int) -> "Config":

@@ -67,6 +67,7 @@ class PfConfig(ctypes.Structure):
         ("slot_pad", ctypes.c_int32),
         ("mlp_kmin", ctypes.c_float), ("mlp_kmax", ctypes.c_float),
         ("derived", ctypes.c_int32),
+        ("tile_x", ctypes.c_int32),
     ]
 
 

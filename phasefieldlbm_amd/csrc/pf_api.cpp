@@ -153,15 +153,23 @@ void fill_constants(pf_ctx* c) {
 // 256x256x1024): FP32 is fastest with ~16k workgroups (z-chunk 8), the
 // binary16 form (3 workgroups per CU) with ~8k (z-chunk 16); short chunks
 // re-load more halo planes, long ones leave the last wave of workgroups
-// unbalanced.
+// unbalanced.  Small grids get the longest chunk, down to 1, that still
+// gives every one of the 256 CUs a workgroup.  Measured on 64 x 8 tiles:
+// 64^3 droplet 14.7k MLUPS at z-chunk 1 (496 workgroups), 13.8k at 2 (248),
+// 10.9k at 4 (128); the reference's 64x64x128 D3Q19 jet 14.2k at 4 (256),
+// 14.0k at 2 (504), 12.9k at 1.
+int tile_x(const pf_config& g) { return g.tile_x > 0 ? g.tile_x : pf::tile_width(g.nx); }
+
 int choose_zchunk(const pf_config& g) {
     if (g.zchunk > 0) return g.zchunk;
-    const int ntx = (g.nx + pf::kTX - 1) / pf::kTX, nty = (g.ny + pf::kTY - 1) / pf::kTY;
-    const long cols = (long)ntx * nty;
+    const int tw = tile_x(g), th = pf::kThreads / tw;
+    const long cols = (long)((g.nx + tw - 1) / tw) * ((g.ny + th - 1) / th);
     const long target = g.storage == PF_STORE_FP16 ? 8192 : 16384;
     long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;
+    constexpr long kCUs = 256;
+    while (zc > 1 && ((long)(g.nz - 2) + zc - 1) / zc * cols < kCUs) --zc;
     return (int)zc;
 }
 
@@ -244,6 +252,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     if (g.noise < 0 || g.noise > 2) return PF_ERR_CONFIG;
     if (g.kernel != PF_KERNEL_PAIR && g.kernel != PF_KERNEL_LITERAL) return PF_ERR_CONFIG;
     if (g.derived & ~(PF_DERIVED_AVERAGE | PF_DERIVED_REYNOLDS | PF_DERIVED_VORTICITY)) return PF_ERR_CONFIG;
+    if (g.tile_x != 0 && g.tile_x != pf::kTX && g.tile_x != pf::kTXNarrow) return PF_ERR_CONFIG;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
     if (N >= (size_t)1 << 30) return PF_ERR_CONFIG;  // byte offsets of one slot stay below 2^32
 
@@ -337,8 +346,9 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     k.plane_f = pitch_f * (uint64_t)g.ny;
     k.elems_f = c->f_elems;
     k.zchunk = choose_zchunk(g);
-    k.ntx = (g.nx + pf::kTX - 1) / pf::kTX;
-    k.nty = (g.ny + pf::kTY - 1) / pf::kTY;
+    k.tile_x = tile_x(g);
+    k.ntx = (g.nx + k.tile_x - 1) / k.tile_x;
+    k.nty = (g.ny + pf::kThreads / k.tile_x - 1) / (pf::kThreads / k.tile_x);
     k.ntz = (g.nz - 2 + k.zchunk - 1) / k.zchunk;
     fill_constants(c);
     *out = c;

@@ -19,7 +19,8 @@
 // for the droplet.  The jet's z = 0 and z = nz-1 ghost planes carry the
 // populations the boundary kernel supplies for the next step.
 //
-// k_step marches a 128 x 4 column of cells through z.  A ring of the phi tile
+// k_step marches a 128 x 4 column of cells (64 x 8 on grids at most 64
+// wide) through z.  A ring of the phi tile
 // (5 planes in FP32, 4 in binary16) with a 2-cell halo and a 3-plane ring of
 // normals with a 1-cell halo live in LDS, so phi is read from HBM about once
 // per cell and the normals never leave the chip (the reference stores
@@ -36,10 +37,6 @@
 
 namespace pf {
 
-constexpr int kHX = kTX + 4;  // phi tile with 2-cell halo
-constexpr int kHY = kTY + 4;
-constexpr int kNX1 = kTX + 2;  // normal tile with 1-cell halo
-constexpr int kNY1 = kTY + 2;
 
 // ---------------------------------------------------------------------------
 // population storage
@@ -397,33 +394,36 @@ __device__ __forceinline__ void pair_out(const Coll& k, float ux, float uy, floa
 #define PF_SLIM_LDS_FP16 1
 #endif
 
-template <int Q, bool JET, bool HALF, int PAR, bool MLP, bool LIT, bool SAVE>
+template <int Q, bool JET, bool HALF, int PAR, bool MLP, bool LIT, bool SAVE, int TX>
 __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
+    constexpr int TY = kThreads / TX;                  // rows of the TX x TY tile
+    constexpr int HX = TX + 4, HY = TY + 4;            // phi tile with 2-cell halo
+    constexpr int NX1 = TX + 2, NY1 = TY + 2;          // normal tile with 1-cell halo
     constexpr bool SLIM = HALF ? (PF_SLIM_LDS_FP16 != 0) : (PF_SLIM_LDS_FP32 != 0);
     constexpr int kRing = SLIM ? 4 : 5;  // phi planes
     constexpr int kNC = SLIM ? 3 : 4;    // components per normal plane
-    __shared__ float s_phi[kRing][kHY * kHX];
-    __shared__ float s_n[3][kNC][kNY1 * kNX1];
+    __shared__ float s_phi[kRing][HY * HX];
+    __shared__ float s_n[3][kNC][NY1 * NX1];
     __shared__ float s_ind[SLIM ? 2 : 1][SLIM ? kThreads : 1];
     constexpr int NP = (Q - 1) / 2;
 
     const int tid = threadIdx.x;
-    const int tx = tid & (kTX - 1), ty = tid / kTX;
+    const int tx = tid & (TX - 1), ty = tid / TX;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
     const int bx = (int)(lb % (uint32_t)p.ntx);
     const uint32_t rem = lb / (uint32_t)p.ntx;
     const int by = (int)(rem % (uint32_t)p.nty);
     const int bz = (int)(rem / (uint32_t)p.nty);
-    const int x0 = bx * kTX, y0 = by * kTY;
+    const int x0 = bx * TX, y0 = by * TY;
     const int zs = 1 + bz * p.zchunk;
     const int ze = min(zs + p.zchunk, nz - 1);
     const float* __restrict__ phi_g = p.fld[10];
 
     auto load_phi = [&](int pz) {
         float* dst = s_phi[(pz + kRing) % kRing];
-        for (int i = tid; i < kHX * kHY; i += kThreads) {
-            const int lx = i % kHX, ly = i / kHX;
+        for (int i = tid; i < HX * HY; i += kThreads) {
+            const int lx = i % HX, ly = i / HX;
             const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
             float v = 0.0f;
             if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && pz >= 0 && pz < nz)
@@ -440,15 +440,15 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         const float* pm = s_phi[(pz + kRing - 1) % kRing];
         const float* p0 = s_phi[pz % kRing];
         const float* pp = s_phi[(pz + 1) % kRing];
-        for (int i = tid; i < kNX1 * kNY1; i += kThreads) {
-            const int lx = i % kNX1, ly = i / kNX1;
+        for (int i = tid; i < NX1 * NY1; i += kThreads) {
+            const int lx = i % NX1, ly = i / NX1;
             const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
             float nrx = 0.0f, nry = 0.0f, nrz = 0.0f, ind = 0.0f;
             if (gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2 && pz >= 1 && pz <= nz - 2) {
-                const int o = (ly + 1) * kHX + (lx + 1);
+                const int o = (ly + 1) * HX + (lx + 1);
                 auto val = [&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    constexpr int off = kCY[q] * kHX + kCX[q];
+                    constexpr int off = kCY[q] * HX + kCX[q];
                     const float* pl = kCZ[q] < 0 ? pm : (kCZ[q] > 0 ? pp : p0);
                     return pl[o + off];
                 };
@@ -465,7 +465,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             s_n[r][1][i] = nry;
             s_n[r][2][i] = nrz;
             if constexpr (SLIM) {
-                if (lx >= 1 && lx <= kTX && ly >= 1 && ly <= kTY) s_ind[pz & 1][(ly - 1) * kTX + (lx - 1)] = ind;
+                if (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY) s_ind[pz & 1][(ly - 1) * TX + (lx - 1)] = ind;
             } else {
                 s_n[r][3][i] = ind;
             }
@@ -490,14 +490,14 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         __syncthreads();
 
         // ---- curvature and continuum surface force (src/phaseField.cuh:142-219)
-        const int o = (ty + 1) * kNX1 + (tx + 1);
+        const int o = (ty + 1) * NX1 + (tx + 1);
         const float* nm[3] = {s_n[(z + 2) % 3][0], s_n[z % 3][0], s_n[(z + 1) % 3][0]};
         auto nval = [&](int comp_, auto qc) {
             constexpr int q = decltype(qc)::value;
-            constexpr int off = kCY[q] * kNX1 + kCX[q];
-            return nm[kCZ[q] + 1][comp_ * (kNY1 * kNX1) + o + off];
+            constexpr int off = kCY[q] * NX1 + kCX[q];
+            return nm[kCZ[q] + 1][comp_ * (NY1 * NX1) + o + off];
         };
-        const float phi = s_phi[z % kRing][(ty + 2) * kHX + (tx + 2)];
+        const float phi = s_phi[z % kRing][(ty + 2) * HX + (tx + 2)];
         float curvature = 0.0f;
         if (active) {
             const float scx = iso_sum<Q, 0>([&](auto qc) { return nval(0, qc); });
@@ -909,10 +909,17 @@ static hipError_t phase_sum_t(const KParams& p, hipStream_t s) {
 template <int Q, bool JET, bool HALF, bool MLP, bool LIT, bool SAVE>
 static hipError_t step_t(const KParams& p, int parity, hipStream_t s) {
     const uint32_t nblk = (uint32_t)p.ntx * (uint32_t)p.nty * (uint32_t)p.ntz;
-    if (parity)
-        hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE>), dim3(nblk), dim3(kThreads), 0, s, p);
-    else
-        hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE>), dim3(nblk), dim3(kThreads), 0, s, p);
+    if (p.tile_x == kTXNarrow && kTXNarrow != kTX) {
+        if (parity)
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE, kTXNarrow>), dim3(nblk), dim3(kThreads), 0, s, p);
+        else
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE, kTXNarrow>), dim3(nblk), dim3(kThreads), 0, s, p);
+    } else {
+        if (parity)
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE, kTX>), dim3(nblk), dim3(kThreads), 0, s, p);
+        else
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE, kTX>), dim3(nblk), dim3(kThreads), 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ def test_header_symbols_exported():
 
 
 def test_config_struct_layout():
-    # the ctypes mirror has the C size: 28 four-byte members
+    # the ctypes mirror has the C size: four-byte members only, in header order
     assert ctypes.sizeof(_abi.PfConfig) == 4 * len(_abi.PfConfig._fields_)
     text = open(os.path.join(ROOT, "include", "phasefieldlbm.h")).read()
     body = text[text.index("typedef struct pf_config {"):text.index("} pf_config;")]
@@ -48,7 +48,7 @@ def test_reference_defaults():
 
 
 @pytest.mark.parametrize("field,value", [("velocity_set", 15), ("flow_case", 7), ("storage", 9), ("nx", 2),
-                                         ("noise", 5), ("interface_width", 0.0)])
+                                         ("noise", 5), ("interface_width", 0.0), ("tile_x", 32)])
 def test_bad_config_rejected(field, value):
     cfg = pf.presets.static_droplet(n=16)
     setattr(cfg, field, value)

@@ -284,6 +284,38 @@ def test_zchunk_invariance(zchunk, storage):
             np.testing.assert_array_equal(a.field(k), b.field(k))
 
 
+@pytest.mark.parametrize("storage", [0, STORE_FP16])
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+@pytest.mark.parametrize("case", ["droplet", "jet"])
+def test_tile_width_invariance(case, Q, storage):
+    """The 128 x 4 tile (grids wider than 64) and the 64 x 8 tile (which the
+    automatic choice gives the small grids of the oracle comparisons above)
+    produce every field bit for bit alike, in the pair and the literal
+    collision and with the learned closure.  This carries those comparisons
+    over to the 128-wide tile."""
+    rng = np.random.default_rng(11)
+    params = (rng.normal(size=6017) * 0.1).astype(np.float32)
+
+    def run(tile, kernel, closure):
+        cfg = (presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=storage) if case == "droplet"
+               else presets.round_jet(24, 24, 48, 6, Q, storage=storage, noise=NOISE_UNIFORM_Z))
+        cfg.tile_x, cfg.kernel = tile, kernel
+        if closure:
+            cfg.closure = pf.CLOSURE_MLP
+            cfg.mlp_kmin, cfg.mlp_kmax = -100.0, 100.0
+        with pf.Simulation(cfg) as sim:
+            if closure:
+                sim.set_closure_params(params)
+            sim.init().step(21, save=True)
+            return {k: sim.field(k) for k in pf.FIELDS}
+
+    print(f"\n[tile] case={case} Q={Q} storage={storage}", flush=True)
+    for kernel, closure in ((pf.KERNEL_PAIR, False), (pf.KERNEL_LITERAL, False), (pf.KERNEL_PAIR, True)):
+        a, b = run(64, kernel, closure), run(128, kernel, closure)
+        for k in pf.FIELDS:
+            assert np.array_equal(a[k], b[k]), (kernel, closure, k)
+
+
 def test_slot_padding_invariance():
     """The padded population slot stride changes addresses, not results."""
     cfg = presets.round_jet(20, 20, 40, 6, D3Q19, noise=NOISE_UNIFORM_Z)

@@ -150,10 +150,11 @@ void fill_constants(pf_ctx* c) {
 }
 
 // z cells marched per workgroup.  Measured on MI355X (D3Q27 jet
-// 256x256x1024): FP32 is fastest with ~16k workgroups (z-chunk 8), the
-// binary16 form (3 workgroups per CU) with ~8k (z-chunk 16); short chunks
-// re-load more halo planes, long ones leave the last wave of workgroups
-// unbalanced.  Small grids get the longest chunk, down to 1, that still
+// 256x256x1024, tile rows dealt to the XCDs, 3 runs each): FP32 is fastest
+// with ~32k workgroups, k_step 3.05 ms at z-chunk 3 or 4 against 3.10 at 5,
+// 3.12 at 2 and 3.13 at 8; the binary16 form (3 workgroups per CU) within
+// 1 % from z-chunk 10 to 16 (~8k workgroups at 16).  Short chunks re-load
+// more halo planes, long ones leave the last wave of workgroups unbalanced.  Small grids get the longest chunk, down to 1, that still
 // gives every one of the 256 CUs a workgroup.  Measured on 64 x 8 tiles:
 // 64^3 droplet 14.7k MLUPS at z-chunk 1 (496 workgroups), 13.8k at 2 (248),
 // 10.9k at 4 (128); the reference's 64x64x128 D3Q19 jet 14.2k at 4 (256),
@@ -164,7 +165,7 @@ int choose_zchunk(const pf_config& g) {
     if (g.zchunk > 0) return g.zchunk;
     const int tw = tile_x(g), th = pf::kThreads / tw;
     const long cols = (long)((g.nx + tw - 1) / tw) * ((g.ny + th - 1) / th);
-    const long target = g.storage == PF_STORE_FP16 ? 8192 : 16384;
+    const long target = g.storage == PF_STORE_FP16 ? 8192 : 32768;
     long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;

@@ -154,8 +154,9 @@ void fill_constants(pf_ctx* c) {
 // with ~32k workgroups, k_step 3.05 ms at z-chunk 3 or 4 against 3.10 at 5,
 // 3.12 at 2 and 3.13 at 8; the binary16 form (3 workgroups per CU) within
 // 1 % from z-chunk 10 to 16 (~8k workgroups at 16).  Short chunks re-load
-// more halo planes, long ones leave the last wave of workgroups unbalanced.  Small grids get the longest chunk, down to 1, that still
-// gives every one of the 256 CUs a workgroup.  Measured on 64 x 8 tiles:
+// more halo planes, long ones leave the last wave of workgroups unbalanced.
+// Small grids get the longest chunk, down to 1, that still gives every one
+// of the 256 CUs a workgroup.  Measured on 64 x 8 tiles:
 // 64^3 droplet 14.7k MLUPS at z-chunk 1 (496 workgroups), 13.8k at 2 (248),
 // 10.9k at 4 (128); the reference's 64x64x128 D3Q19 jet 14.2k at 4 (256),
 // 14.0k at 2 (504), 12.9k at 1.

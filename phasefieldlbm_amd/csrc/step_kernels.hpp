@@ -20,11 +20,11 @@
 // populations the boundary kernel supplies for the next step.
 //
 // k_step marches a 128 x 4 column of cells (64 x 8 on grids at most 64
-// wide) through z.  A ring of the phi tile
-// (5 planes in FP32, 4 in binary16) with a 2-cell halo and a 3-plane ring of
-// normals with a 1-cell halo live in LDS, so phi is read from HBM about once
-// per cell and the normals never leave the chip (the reference stores
-// normx/y/z, ind and the force in HBM between three kernels).
+// wide) through z.  A ring of the phi tile (5 planes in FP32, 4 in
+// binary16) with a 2-cell halo and a 3-plane ring of normals with a 1-cell
+// halo live in LDS, so phi is read from HBM about once per cell and the
+// normals never leave the chip (the reference stores normx/y/z, ind and the
+// force in HBM between three kernels).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,7 +36,6 @@
 #include "phase_flux.hpp"
 
 namespace pf {
-
 
 // ---------------------------------------------------------------------------
 // population storage

@@ -290,7 +290,8 @@ def test_rows_of_tiles(Q, storage):
     """A grid 136 wide has two or three k_step tiles per row, so workgroups
     take whole rows of tiles round-robin over the XCDs, and the launch ends
     in an incomplete round that keeps its order (380 workgroups, 368 dealt).
-    Against the oracle, and bit for bit across tile widths and z-chunks."""
+    Against the oracle, and bit for bit across tile widths and z-chunks, in
+    the pair and the literal collision and with the learned closure."""
     def mk():
         return presets.round_jet(136, 20, 40, 6, Q, storage=storage, noise=NOISE_UNIFORM_Z)
     if storage == STORE_FP16:
@@ -298,18 +299,26 @@ def test_rows_of_tiles(Q, storage):
     else:
         compare(mk(), 24)
 
-    def run(tile, zchunk):
+    params = (np.random.default_rng(12).normal(size=6017) * 0.1).astype(np.float32)
+
+    def run(tile, zchunk, kernel, closure):
         cfg = mk()
-        cfg.tile_x, cfg.zchunk = tile, zchunk
+        cfg.tile_x, cfg.zchunk, cfg.kernel = tile, zchunk, kernel
+        if closure:
+            cfg.closure = pf.CLOSURE_MLP
+            cfg.mlp_kmin, cfg.mlp_kmax = -100.0, 100.0
         with pf.Simulation(cfg) as sim:
+            if closure:
+                sim.set_closure_params(params)
             sim.init().step(24, save=True)
             return {k: sim.field(k) for k in pf.FIELDS}
 
-    ref = run(128, 0)
-    for tile, zchunk in ((64, 0), (128, 4), (64, 3)):
-        got = run(tile, zchunk)
-        for k in pf.FIELDS:
-            assert np.array_equal(got[k], ref[k]), (tile, zchunk, k)
+    for kernel, closure in ((pf.KERNEL_PAIR, False), (pf.KERNEL_LITERAL, False), (pf.KERNEL_PAIR, True)):
+        ref = run(128, 0, kernel, closure)
+        for tile, zchunk in ((64, 0), (128, 4), (64, 3)):
+            got = run(tile, zchunk, kernel, closure)
+            for k in pf.FIELDS:
+                assert np.array_equal(got[k], ref[k]), (kernel, closure, tile, zchunk, k)
 
 
 @pytest.mark.parametrize("storage", [0, STORE_FP16])

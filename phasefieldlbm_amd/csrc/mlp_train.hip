@@ -301,9 +301,10 @@ __global__ __launch_bounds__(64) void k_mfma_layout(const uint16_t* a16, const u
 // ---------------------------------------------------------------------------
 constexpr int kClZ = 8;
 
-// Launch bound: 4 waves per SIMD.  The compiler then keeps 128 VGPRs and
-// spills the rest (mostly loop-invariant weight operands) to scratch; measured
-// on MI355X 5 % faster than 2 waves per SIMD with 256 VGPRs and no spills.
+// Launch bound: 4 waves per SIMD, 128 VGPRs.  With the weight fragments
+// read inside the job loop (mlp_eval16) only 8 VGPRs spill.  When they were
+// hoisted (94 spilled), this bound was still 5 % faster than 2 waves per SIMD
+// with 256 VGPRs and no spills.
 __global__ __launch_bounds__(256, 4) void k_closure(KParams p) {
     __shared__ MlpWeightsLds W;
     constexpr int kPX = 66, kPY = 6;  // 64 x 4 tile with a 1-cell halo

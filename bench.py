@@ -41,15 +41,15 @@ def kernel_bytes_per_cell(q, s):
     return 2 * q * s + 3 * s + 4
 
 
-def run_form(pf, cfg, steps, warmup, closure_params=None):
+def run_form(pf, cfg, steps, warmup, closure_params=None, graph=False):
     sim = pf.Simulation(cfg)
     if closure_params is not None:
         sim.set_closure_params(closure_params)
     sim.init()
-    sim.step(warmup)
+    sim.step(warmup, graph=graph)
     sim.sync()
     t0 = time.perf_counter()
-    sim.step(steps)
+    sim.step(steps, graph=graph)
     sim.sync()
     wall = time.perf_counter() - t0
     tm = sim.time_steps(steps)  # HIP events on the simulation's stream
@@ -226,6 +226,19 @@ def main():
             others[name] = round(c.nx * c.ny * c.nz * args.steps / w / 1e6, 1)
         out["other_workloads_mlups"] = others
         log(f"[bench] other workloads: {others}")
+        # the step replayed as a captured graph per parity (the reference's
+        # execution model) against direct launches, on the small grids where
+        # launch overhead could show; wall clock per step
+        graph_ms = {}
+        for name, make in (("droplet_d3q19_64", presets.WORKLOADS["droplet_d3q19_64"][0]),
+                           ("reference_jet_d3q19_64x64x128", lambda: presets.reference_jet(pf.D3Q19))):
+            row = {}
+            for tag, g in (("direct", False), ("graph", True)):
+                w, _, _ = run_form(pf, make(), args.steps, args.warmup, graph=g)
+                row[tag] = round(1e3 * w / args.steps, 4)
+            graph_ms[name] = row
+        out["graph_vs_direct_ms_per_step"] = graph_ms
+        log(f"[bench] graph vs direct ms/step: {graph_ms}")
         # the reference's own default jet (64x64x128, D = 10, Gaussian inflow
         # noise), at which its README quotes 470 (D3Q19) and 410 (D3Q27)
         # MLUPS on an RTX 3050, probably with binary16 f; context only

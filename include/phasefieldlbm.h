@@ -101,6 +101,16 @@ enum pf_status {
 
 /* pf_step flags */
 #define PF_STEP_SAVE_LAST 1u /* materialise every pf_field on the last step */
+/* Replay the step's kernels before the boundary conditions (phase sum,
+ * closure, collide-and-stream) as a captured hipGraph, one per A-A parity,
+ * as the reference replays its CUDA graph every step
+ * (src/cuda/CUDAGraph.cuh:34-58, src/main.cu:163-172).  The boundary
+ * kernels, which take the time index, are launched after it as in the
+ * reference.  Results are bit-identical to direct launches.  A graph is
+ * captured on first use and again whenever a kernel parameter it holds
+ * changes; the first step after the populations are set and steps that
+ * materialise the fields launch directly. */
+#define PF_STEP_GRAPH 2u
 
 /*
  * Run configuration.  The reference keeps these as constexpr values

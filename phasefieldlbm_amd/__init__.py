@@ -168,10 +168,15 @@ class Simulation:
         self._check(self._lib.pf_init_distros(self._h), "pf_init_distros")
         self.t = 0
 
-    def step(self, n: int = 1, save: bool = False, t0: Optional[int] = None):
-        """Advance n timesteps; the inflow noise uses time index t0+s."""
+    def step(self, n: int = 1, save: bool = False, t0: Optional[int] = None, graph: bool = False):
+        """Advance n timesteps; the inflow noise uses time index t0+s.
+
+        graph=True replays the kernels before the boundary conditions as a
+        captured hipGraph per A-A parity (bit-identical to direct launches).
+        """
         t0 = self.t if t0 is None else t0
-        self._check(self._lib.pf_step(self._h, t0, n, _abi.STEP_SAVE_LAST if save else 0), "pf_step")
+        flags = (_abi.STEP_SAVE_LAST if save else 0) | (_abi.STEP_GRAPH if graph else 0)
+        self._check(self._lib.pf_step(self._h, t0, n, flags), "pf_step")
         self.t = t0 + n
         return self
 

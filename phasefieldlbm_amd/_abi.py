@@ -20,6 +20,7 @@ NOISE_GAUSSIAN_XY, NOISE_UNIFORM_Z, NOISE_NONE = 0, 1, 2
 CLOSURE_FD, CLOSURE_MLP = 0, 1
 KERNEL_PAIR, KERNEL_LITERAL = 0, 1
 STEP_SAVE_LAST = 1
+STEP_GRAPH = 2
 OUT_BIN, OUT_VTI, OUT_VTR = 1, 2, 4
 MLP_PARAMS = 6017
 

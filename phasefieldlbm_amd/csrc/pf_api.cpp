@@ -47,6 +47,11 @@ struct pf_ctx {
     int parity = 0;
     bool fresh = false;  // populations just set: the next phase sum is the local one
     bool inited = false;
+    // PF_STEP_GRAPH: one captured graph per A-A parity (phase sum, closure,
+    // step), valid while the kernel parameters it was captured with hold.
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    pf::KParams gkey[2]{};
+    int gvariant[2] = {-1, -1};
     size_t bytes = 0;
     std::string err;
 };
@@ -374,6 +379,8 @@ void pf_destroy(pf_ctx* c) {
     for (auto* p : c->der) (void)hipFree(p);
     (void)hipFree(c->mlp_w);
     (void)hipFree(c->mlp_b);
+    for (auto& g : c->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -459,6 +466,61 @@ static void swap_phi(pf_ctx* c) {
     c->kp.phi_next = c->phi_next;
 }
 
+// The kernels of one step before the boundary conditions: the phase sum, the
+// learned closure when on, and the collide-and-stream of this parity.
+static int launch_core(pf_ctx* c, int variant, hipEvent_t* ev) {
+    if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
+    PF_HIP(c, pf::launch_phase_sum(c->kp, c->jet, c->fresh, c->half, c->stream));
+    swap_phi(c);
+    c->fresh = false;
+    if (variant == 2) PF_HIP(c, pf::launch_closure(c->kp, c->stream));
+    if (ev) PF_HIP(c, hipEventRecord(ev[1], c->stream));
+    PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
+    if (ev) PF_HIP(c, hipEventRecord(ev[2], c->stream));
+    return PF_OK;
+}
+
+// launch_core through a graph, as the reference replays its captured graph
+// every step (src/cuda/CUDAGraph.cuh:34-58, src/main.cu:166).  The graph of a
+// parity is captured on first use and kept while the kernel parameters other
+// than the time index and the save flag (which the graph never sees: it is
+// captured with save = 0 and the inflow kernel runs outside it) equal those
+// it was captured with; phi's two buffers swap every step, so each parity
+// always sees the same pair.  Otherwise it is captured again.
+static int launch_core_graph(pf_ctx* c, int variant) {
+    const int par = c->parity;
+    pf::KParams key;
+    std::memcpy(&key, &c->kp, sizeof key);  // bytewise, padding included, for the memcmp below
+    key.t = 0;
+    if (!c->gexec[par] || c->gvariant[par] != variant || std::memcmp(&key, &c->gkey[par], sizeof key) != 0) {
+        if (c->gexec[par]) {
+            PF_HIP(c, hipGraphExecDestroy(c->gexec[par]));
+            c->gexec[par] = nullptr;
+        }
+        PF_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_core(c, variant, nullptr);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc != PF_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "hipStreamEndCapture");
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[par], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            c->gexec[par] = nullptr;
+            return hip_fail(c, ei, "hipGraphInstantiate");
+        }
+        std::memcpy(&c->gkey[par], &key, sizeof key);
+        c->gvariant[par] = variant;
+        swap_phi(c);  // capture ran nothing: undo its swap, the launch below redoes it
+    }
+    PF_HIP(c, hipGraphLaunch(c->gexec[par], c->stream));
+    swap_phi(c);
+    return PF_OK;
+}
+
 static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hipEvent_t* ev) {
     // Running means need the macroscopic fields of every step (the reference
     // computes them every step and averages after the boundary conditions).
@@ -468,14 +530,16 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
         c->kp.t = (uint32_t)(t0 + s);
         c->kp.save = (avg || ((flags & PF_STEP_SAVE_LAST) && s + 1 == nsteps)) ? 1 : 0;
         const int variant = c->cfg.closure == PF_CLOSURE_MLP ? 2 : c->cfg.kernel;
-        if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
-        PF_HIP(c, pf::launch_phase_sum(c->kp, c->jet, c->fresh, c->half, c->stream));
-        swap_phi(c);
-        c->fresh = false;
-        if (variant == 2) PF_HIP(c, pf::launch_closure(c->kp, c->stream));
-        if (ev) PF_HIP(c, hipEventRecord(ev[1], c->stream));
-        PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
-        if (ev) PF_HIP(c, hipEventRecord(ev[2], c->stream));
+        // The first step after the populations are set runs the local phase
+        // sum once, and save steps run the materialising kernel variant:
+        // both are launched directly.
+        if ((flags & PF_STEP_GRAPH) && !ev && !c->fresh && !c->kp.save) {
+            const int rc = launch_core_graph(c, variant);
+            if (rc != PF_OK) return rc;
+        } else {
+            const int rc = launch_core(c, variant, ev);
+            if (rc != PF_OK) return rc;
+        }
         if (c->jet) PF_HIP(c, pf::launch_jet_bc(c->kp, c->Q, c->half, c->parity ^ 1, c->stream));
         if (avg) PF_HIP(c, pf::launch_average(c->kp, dp, (float)(t0 + s + 1), c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[3], c->stream));

@@ -422,3 +422,44 @@ def test_baseline_static_droplet_2000_steps():
         dps.append((rho[r < R - 4].mean() - rho[(r > R + 4) & inner].mean()) / 3.0)
         assert abs(dps[-1] - 2 * cfg.sigma / R) / (2 * cfg.sigma / R) < 0.10, dps
     assert abs(dps[0] - dps[1]) < 1e-3 * abs(dps[1]), dps
+
+
+def _graph_case(case):
+    if case == "droplet":
+        return presets.static_droplet(n=32, radius=8.0, velocity_set=D3Q19)
+    if case == "jet_fp16":
+        return presets.round_jet(24, 24, 40, 6, D3Q27, storage=STORE_FP16, noise=NOISE_UNIFORM_Z)
+    cfg = presets.round_jet(24, 24, 40, 6, D3Q27, noise=NOISE_UNIFORM_Z)
+    if case == "jet_closure":
+        cfg.closure = pf.CLOSURE_MLP
+    return cfg
+
+
+@pytest.mark.parametrize("case", ["droplet", "jet", "jet_fp16", "jet_closure"])
+def test_graph_replay_matches_direct_launches(case):
+    """PF_STEP_GRAPH replays the phase sum, closure and step of each parity as
+    a captured graph; the inflow kernel, which takes the time index, runs
+    outside it.  Every field equals direct launches bit for bit, over many
+    replays of both parities with eager work (diagnostics, field reads)
+    between them, across a re-initialisation, and on odd and even step
+    counts (each call ends on a save step, which launches directly)."""
+    from phasefieldlbm_amd import closure
+    cfg = _graph_case(case)
+    params = closure.load() if case == "jet_closure" else None
+    with pf.Simulation(cfg) as a, pf.Simulation(cfg) as b:
+        for s in (a, b):
+            if params is not None:
+                s.set_closure_params(params)
+            s.init()
+        for n in (1, 6, 9, 2):
+            a.step(n, save=True)
+            b.step(n, save=True, graph=True)
+            da, db = a.diagnostics(), b.diagnostics()
+            for k in da:
+                np.testing.assert_array_equal(da[k], db[k], err_msg=f"{case} n={n} diagnostics {k}")
+            for k in CORE + ("pxx", "pxz", "fsx"):
+                np.testing.assert_array_equal(a.field(k), b.field(k), err_msg=f"{case} n={n} {k}")
+        a.init().step(5).step(1, save=True)
+        b.init().step(5, graph=True).step(1, save=True, graph=True)
+        for k in CORE:
+            np.testing.assert_array_equal(a.field(k), b.field(k), err_msg=f"{case} after re-init {k}")

@@ -137,13 +137,15 @@ def test_simulation_write_matches_fields(tmp_path):
 
 
 @pytest.mark.gpu
-def test_driver_droplet_run(tmp_path):
+@pytest.mark.parametrize("graph", ["off", "on"])
+def test_driver_droplet_run(tmp_path, graph):
     """pfsim DROPLET D3Q19 <ID>: directory layout, info file, output cadence
     (steps 0, 10, 20 for NSTEPS=20, MACRO_SAVE=10) and the final fields equal
-    those of the same 21 steps run through the Python binding."""
+    those of the same 21 steps run through the Python binding, with the step
+    launched directly or replayed as a captured graph (--graph)."""
     import phasefieldlbm_amd as pf
     r = _run(["DROPLET", "D3Q19", "007", "--preset", "droplet_d3q19_64", "--steps", "20", "--save", "10",
-              "--fields", "rho,uz,phi", "--format", "bin,vti,vtr"], tmp_path)
+              "--fields", "rho,uz,phi", "--format", "bin,vti,vtr", "--graph", graph], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     d = tmp_path / "bin" / "DROPLET" / "D3Q19" / "007"
     info = (d / "007_info.txt").read_text()

@@ -436,16 +436,35 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     const int ze = min(zs + p.zchunk, nz - 1);
     const float* __restrict__ phi_g = p.fld[10];
 
-    auto load_phi = [&](int pz) {
-        float* dst = s_phi[(pz + kRing) % kRing];
-        for (int i = tid; i < HX * HY; i += kThreads) {
-            const int lx = i % HX, ly = i / HX;
-            const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
-            float v = 0.0f;
-            if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && pz >= 0 && pz < nz)
-                v = phi_g[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)pz)];
-            dst[i] = v;
+    // phi of plane pz, tile + 2-cell halo, zero outside the grid, into ring
+    // slot pz % kRing.  fetch_phi issues all of a thread's loads at once, at
+    // addresses clamped into the grid; put_phi applies the bounds as it
+    // stores to LDS.  (Loads under per-element bounds tests compiled to a
+    // load and a wait each.)
+    constexpr int kPhiIt = (HX * HY + kThreads - 1) / kThreads;
+    auto fetch_phi = [&](int pz, float (&v)[kPhiIt]) {
+        const int cz = min(max(pz, 0), nz - 1);
+#pragma unroll
+        for (int j = 0; j < kPhiIt; ++j) {
+            const int i = min(tid + j * kThreads, HX * HY - 1);
+            const int gx = min(max(x0 - 2 + i % HX, 0), nx - 1), gy = min(max(y0 - 2 + i / HX, 0), ny - 1);
+            v[j] = phi_g[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)cz)];
         }
+    };
+    auto put_phi = [&](int pz, const float (&v)[kPhiIt]) {
+        float* dst = s_phi[(pz + kRing) % kRing];
+        const bool zin = pz >= 0 && pz < nz;
+#pragma unroll
+        for (int j = 0; j < kPhiIt; ++j) {
+            const int i = tid + j * kThreads;
+            const int gx = x0 - 2 + i % HX, gy = y0 - 2 + i / HX;
+            if (i < HX * HY) dst[i] = (zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? v[j] : 0.0f;
+        }
+    };
+    auto load_phi = [&](int pz) {
+        float v[kPhiIt];
+        fetch_phi(pz, v);
+        put_phi(pz, v);
     };
 
     // normals and |grad phi| on the 1-halo tile of plane pz
@@ -488,8 +507,14 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         }
     };
 
-    // prologue
-    for (int pz = zs - 2; pz <= zs + 1; ++pz) load_phi(pz);
+    // prologue: the four planes' loads all issued before any is stored
+    {
+        float v[4][kPhiIt];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fetch_phi(zs - 2 + k, v[k]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) put_phi(zs - 2 + k, v[k]);
+    }
     __syncthreads();
     comp_normals(zs - 1);
     comp_normals(zs);

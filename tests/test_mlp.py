@@ -98,17 +98,21 @@ def test_mlp_training_fits():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("grid", [(32, 32, 32), (70, 30, 37)])
 @pytest.mark.parametrize("storage", [0, 1])
-def test_closure_in_timestep(storage):
+def test_closure_in_timestep(storage, grid):
     """k_closure + the closure variant of the fused step: the curvature the
     step applies at interface cells equals the reference network evaluated
     on the same phi stencils (and their point reflections).  Both storage
-    forms (binary16 uses the step's slim LDS layout)."""
+    forms (binary16 uses the step's slim LDS layout).  The 70 x 30 x 37 box
+    gives k_closure's 64 x 4 x 8 workgroups a second, partial x tile, a
+    partial row block and a last z-chunk of 3 planes."""
     import phasefieldlbm_amd as pf
     from phasefieldlbm_amd import closure, presets
     rng = np.random.default_rng(5)
     p = (rng.normal(size=6017) * 0.1).astype(np.float32)
     cfg = presets.static_droplet(n=32, radius=8.0, storage=storage)
+    cfg.nx, cfg.ny, cfg.nz = grid
     cfg.closure = pf.CLOSURE_MLP
     cfg.mlp_kmin, cfg.mlp_kmax = -100.0, 100.0  # network at every interface cell
     with pf.Simulation(cfg) as sim:

@@ -37,6 +37,18 @@
 
 namespace pf {
 
+// Workgroup barrier for LDS only: the LDS writes before it are complete and
+// visible after it, while global loads and stores stay in flight.  k_step
+// passes nothing between threads through global memory (phi is read-only in
+// it, and under A-A addressing each location a cell reads is written only by
+// that cell), so its barriers need not drain the population stores of the
+// previous plane, which __syncthreads' full fence does.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---------------------------------------------------------------------------
 // population storage
 // ---------------------------------------------------------------------------
@@ -510,11 +522,11 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) put_phi(zs - 2 + k, v[k]);
     }
-    __syncthreads();
+    lds_barrier();
     comp_normals(zs - 1);
     comp_normals(zs);
     // the 4-plane ring reuses the slot of phi(zs - 2) for the first load below
-    if constexpr (SLIM) __syncthreads();
+    if constexpr (SLIM) lds_barrier();
 
     const int x = x0 + tx, y = y0 + ty;
     const bool active = (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2);
@@ -535,9 +547,9 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             fetch_phi(z + 2, phi_next);
             put_phi(z + 2, phi_next);
         }
-        __syncthreads();
+        lds_barrier();
         comp_normals(z + 1);
-        __syncthreads();
+        lds_barrier();
 
         // ---- curvature and continuum surface force (src/phaseField.cuh:142-219)
         const int o = (ty + 1) * NX1 + (tx + 1);

@@ -155,10 +155,13 @@ void fill_constants(pf_ctx* c) {
 }
 
 // z cells marched per workgroup.  Measured on MI355X (D3Q27 jet
-// 256x256x1024, tile rows dealt to the XCDs, 3 runs each): FP32 is fastest
-// with ~32k workgroups, k_step 3.05 ms at z-chunk 3 or 4 against 3.10 at 5,
-// 3.12 at 2 and 3.13 at 8; the binary16 form (3 workgroups per CU) within
-// 1 % from z-chunk 10 to 16 (~8k workgroups at 16).  Short chunks re-load
+// 256x256x1024, tile rows dealt to the XCDs, fresh process per setting, two
+// runs each), with each thread's phi-tile loads issued together: FP32 k_step
+// 2.95-2.97 ms at z-chunk 2 (65k workgroups) against 3.03 at 4, 3.05 at 3
+// and 3.12-3.13 at 1; the binary16 form (3 workgroups per CU) within 0.3 %
+// from z-chunk 4 to 16 (~8k workgroups at 16).  Before those loads were
+// issued together, z-chunk 4 was the FP32 optimum (3.05 ms, 3.12 at 2).
+// The 128x128x512 D3Q19 jet takes 0.294 ms at z-chunk 2 or 4.  Short chunks re-load
 // more halo planes, long ones leave the last wave of workgroups unbalanced.
 // Small grids get the longest chunk, down to 1, that still gives every one
 // of the 256 CUs a workgroup.  Measured on 64 x 8 tiles:
@@ -171,10 +174,14 @@ int choose_zchunk(const pf_config& g) {
     if (g.zchunk > 0) return g.zchunk;
     const int tw = tile_x(g), th = pf::kThreads / tw;
     const long cols = (long)((g.nx + tw - 1) / tw) * ((g.ny + th - 1) / th);
-    const long target = g.storage == PF_STORE_FP16 ? 8192 : 32768;
+    const bool fp16 = g.storage == PF_STORE_FP16;
+    const long target = fp16 ? 8192 : 32768;
     long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;
+    // FP32 grids that reach the target with 4-plane chunks take half the
+    // chunk, at least 2 planes (see the measurements above).
+    if (!fp16 && ((long)(g.nz - 2) + 3) / 4 * cols >= target) zc = std::max(2L, zc / 2);
     constexpr long kCUs = 256;
     while (zc > 1 && ((long)(g.nz - 2) + zc - 1) / zc * cols < kCUs) --zc;
     return (int)zc;

@@ -141,7 +141,7 @@ typedef struct pf_config {
     /* phase field */
     float interface_width;
     float tau_g;           /* only sets the sharpening gamma, as in the reference */
-    float phi0;            /* unused: the D3Q7 populations are not stored (see pf_get_populations) */
+    float reserved0;       /* reserved: must be 0 (pf_create returns PF_ERR_CONFIG otherwise) */
     /* execution */
     int32_t device;        /* HIP device index; <0 reads GPU_INDEX like setDeviceFromEnv */
     int32_t zchunk;        /* z cells marched per workgroup; 0 = automatic */

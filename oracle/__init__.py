@@ -2,7 +2,8 @@
 
 ctypes wrapper of ``libpforacle.so`` (built from ``pf_oracle.cpp`` by the
 Makefile in this directory): a plain C++/OpenMP restatement of the reference
-timestep in FP32, FP32-with-binary16-storage and FP64.  Only tests/,
+timestep in FP32, FP32 with binary16 f (the reference's FP16 mode), FP32
+with binary16 f and phase flux (the library's binary16 form) and FP64.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it, as the
 checker; the product never calls it.
 """
@@ -51,6 +52,10 @@ def load():
         "pfo_f2h": (ctypes.c_uint16, [ctypes.c_float]),
         "pfo_h2f": (ctypes.c_float, [ctypes.c_uint16]),
         "pfo_hash32": (u32, [u32]),
+        "pfo_consts": (i32, [vp, dp]),
+        "pfo_omega": (ctypes.c_double, [vp, ctypes.c_double, i32]),
+        "pfo_omco_zmax": (ctypes.c_double, [vp, ctypes.c_double]),
+        "pfo_sponge_ramp": (ctypes.c_double, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -64,8 +69,14 @@ FIELDS = ("rho", "ux", "uy", "uz", "pxx", "pyy", "pzz", "pxy", "pxz", "pyz",
           "phi", "normx", "normy", "normz", "ind", "fsx", "fsy", "fsz")
 
 
+CONSTS = ("sigma", "gamma", "u_inf", "visc_water", "visc_oil", "tau_water", "tau_oil", "omco_zmin",
+          "omega_ref", "sp_z_start", "sp_inv_nz_m1", "sp_inv_sponge", "cx", "cy", "cz", "R2")
+
+
 class Oracle:
-    """One CPU simulation: precision 32 (FP32), 16 (binary16 storage) or 64."""
+    """One CPU simulation: precision 32 (FP32), 16 (binary16 f, the
+    reference's FP16 mode), 17 (binary16 f and phase flux, the library's
+    binary16 form) or 64."""
 
     def __init__(self, cfg, precision: int = 32):
         self.lib = load()
@@ -106,6 +117,22 @@ class Oracle:
         assert self.lib.pfo_step(self.h, t0, n) == 0
         self.t = t0 + n
         return self
+
+    def consts(self) -> dict:
+        """Derived constants of the configuration (pfo_consts)."""
+        out = np.zeros(len(CONSTS), dtype=np.float64)
+        assert self.lib.pfo_consts(self.h, out.ctypes.data) == 0
+        return dict(zip(CONSTS, out.tolist()))
+
+    def omega(self, phi: float, z: int) -> float:
+        """Collision frequency of a cell with phase phi in plane z."""
+        return float(self.lib.pfo_omega(self.h, float(phi), int(z)))
+
+    def omco_zmax(self, phi: float) -> float:
+        return float(self.lib.pfo_omco_zmax(self.h, float(phi)))
+
+    def sponge_ramp(self, z: int) -> float:
+        return float(self.lib.pfo_sponge_ramp(self.h, int(z)))
 
     def field(self, name) -> np.ndarray:
         out = np.empty(self.cfg.cells, dtype=np.float64)

@@ -6,8 +6,17 @@
 // bench.py's cpu_baseline leg may load it.  Nothing in the product links it.
 //
 // Instantiations: FP32 (parity oracle), FP32 arithmetic with binary16 f
-// storage (precision-form oracle; g stays FP32 as in the reference's
-// precision.cuh) and FP64 (accuracy oracle).
+// storage (the reference's FP16 mode; g stays FP32 as in precision.cuh),
+// FP32 arithmetic with binary16 f and a binary16 phase flux (the library's
+// binary16 form, below), and FP64 (accuracy oracle).
+//
+// The library's binary16 form stores, instead of the seven post-collision g
+// of a cell, its phase flux h = (4 phi u + gamma phi (1 - phi) n) / 8 rounded
+// to binary16; the g arriving at a neighbour are phi/8 +- h_k and the rest
+// population phi/4.  The reference's g have no non-equilibrium part
+// (src/velocitySet/D3Q7.cuh:139-143), so this is the same scheme with one
+// more rounding; the mode streams g formed that way, so its fields can be
+// held to FP32 round-off against the library.
 //
 // Reference files restated here (paths relative to the reference root):
 //   src/phaseField.cuh:31-219        computePhase / computeNormals / computeForces
@@ -27,6 +36,11 @@
 //                                             source cell (the phase-flux identity)
 //   binary16 conversion                       exhaustive round trip + numpy float16
 //   inflow hash / uniform noise               known values of lowbias32 and hand-computed samples
+//   Box-Muller inflow noise                   numpy sqrt(-2 ln u1) cos(2 pi u2) of the same uniforms
+//   jet relaxation, sponge, sigma, gamma      closed forms per configuration (nu = uD/Re,
+//                                             tau = 1/2 + 3 nu, sigma = u^2 D/We, nz/12 sponge
+//                                             cells, cubic smoothstep, K = 100), pfo_consts/pfo_omega
+//   droplet initial condition                 numpy FP64 tanh profile with the axis divisors
 //   full step                                 Laplace law on the FP64 droplet, FP32 vs FP64
 // The reference ships no golden vectors and cannot be built here (it needs
 // nvcc), so no function is pinned against reference outputs directly.
@@ -191,7 +205,7 @@ inline T anti_diffusion(const Lat<T>& G, int q, T sharp, T nx, T ny, T nz) {
 
 // Pair-ordered first moment along `axis` of a value sampled at lattice
 // directions, restricted to weight class k.  The reference writes these sums
-// out by hand (src/phaseField.cuh:331-373, src/lbm.cu:72-80): for each
+// out by hand (src/phaseField.cuh:82-122, :168-210, src/lbm.cu:72-80): for each
 // opposite pair (2j-1, 2j) in increasing j with c_axis != 0, it adds the
 // value at the +1 member and subtracts the one at the -1 member, left to
 // right.  `val(q)` returns the sample for direction q.
@@ -216,7 +230,7 @@ inline T pair_chain(int Q, int axis, int k, V val) {
     return acc;
 }
 
-// Isotropic gradient component (src/phaseField.cuh:331-378): classes 1, 2
+// Isotropic gradient component (src/phaseField.cuh:82-128, :168-213): classes 1, 2
 // (and 3 for D3Q27) each weighted, summed in that order.
 template <class T, class V>
 inline T iso_grad(const Lat<T>& L, int axis, V val) {
@@ -253,6 +267,10 @@ inline T tau_from_nu(T nu, T as2) {
 
 struct Base {
     virtual ~Base() {}
+    virtual void consts(double* out) const = 0;
+    virtual double omega_at(double phi, int z) const = 0;
+    virtual double omco_zmax_at(double phi) const = 0;
+    virtual double ramp_at(int z) const = 0;
     virtual int init() = 0;
     virtual int init_distros() = 0;
     virtual int step(uint64_t t0, uint32_t n) = 0;
@@ -261,7 +279,9 @@ struct Base {
     virtual int get_pop(int which, double* out) = 0;
 };
 
-template <class T, bool HALF>
+// HALF: f stored in binary16.  HALF_H: the phase populations are formed from
+// a binary16 phase flux, as the library's binary16 form does.
+template <class T, bool HALF, bool HALF_H = false>
 struct Sim final : Base {
     using S = typename std::conditional<HALF, uint16_t, T>::type;
     pf_config c;
@@ -500,6 +520,28 @@ struct Sim final : Base {
     T visc_local(T phi) const { return ((T)1 - phi) * k.visc_water + phi * k.visc_oil; }
     T tau_zmax(T phi) const { return tau_from_nu(visc_local(phi) * ((T)1 + (T)100), L.as2); }
     T omco_zmax(T phi) const { return (T)1 - omega_from_nu(visc_local(phi) * ((T)1 + (T)100), L.as2); }
+    // collision frequency of a cell (src/lbm.cu:158-170)
+    T omega_cell(T phi, int z) const {
+        if (!jet) return k.omega_ref;
+        const T tau_phi = ((T)1 - phi) * k.tau_water + phi * k.tau_oil;
+        const T r = sponge_ramp(z);
+        const T tau_eff = tau_phi + r * (tau_zmax(phi) - tau_phi);
+        return (T)1 / tau_eff;
+    }
+
+    // order of pfo_consts
+    void consts(double* o) const override {
+        const double v[] = {(double)k.sigma,     (double)k.gamma,       (double)k.u_inf,
+                            (double)k.visc_water, (double)k.visc_oil,   (double)k.tau_water,
+                            (double)k.tau_oil,   (double)k.omco_zmin,   (double)k.omega_ref,
+                            (double)k.sp_z_start, (double)k.sp_inv_nz_m1, (double)k.sp_inv_sponge,
+                            (double)k.cx,        (double)k.cy,          (double)k.cz,
+                            (double)k.R2};
+        for (size_t i = 0; i < sizeof(v) / sizeof(v[0]); ++i) o[i] = v[i];
+    }
+    double omega_at(double phi, int z) const override { return (double)omega_cell((T)phi, z); }
+    double omco_zmax_at(double phi) const override { return (double)omco_zmax((T)phi); }
+    double ramp_at(int z) const override { return (double)sponge_ramp(z); }
 
     void streamCollide() {  // :130-239
 #pragma omp parallel for schedule(static)
@@ -513,15 +555,7 @@ struct Sim final : Base {
                     const T pxy = F(PF_FIELD_PXY, i), pxz = F(PF_FIELD_PXZ, i), pyz = F(PF_FIELD_PYZ, i);
                     const T fsx = F(PF_FIELD_FSX, i), fsy = F(PF_FIELD_FSY, i), fsz = F(PF_FIELD_FSZ, i);
                     const T phi = F(PF_FIELD_PHI, i);
-                    T omega;
-                    if (jet) {
-                        const T tau_phi = ((T)1 - phi) * k.tau_water + phi * k.tau_oil;
-                        const T r = sponge_ramp(z);
-                        const T tau_eff = tau_phi + r * (tau_zmax(phi) - tau_phi);
-                        omega = (T)1 / tau_eff;
-                    } else {
-                        omega = k.omega_ref;
-                    }
+                    const T omega = omega_cell(phi, z);
                     const T omco = (T)1 - omega;
                     const T uu = (T)1.5 * (ux * ux + uy * uy + uz * uz);
                     for (int q = 0; q < L.Q; ++q) {
@@ -537,14 +571,28 @@ struct Sim final : Base {
                     }
                     const T nrx = F(PF_FIELD_NORMX, i), nry = F(PF_FIELD_NORMY, i), nrz = F(PF_FIELD_NORMZ, i);
                     const T sharp = k.gamma * phi * ((T)1 - phi);
+                    T hflux[3] = {0, 0, 0};
+                    if constexpr (HALF_H) {
+                        // the binary16 phase flux and the g formed from it
+                        const T p4 = (T)4 * phi;
+                        hflux[0] = (T)h2f(f2h((float)((T)0.125 * (p4 * ux + sharp * nrx))));
+                        hflux[1] = (T)h2f(f2h((float)((T)0.125 * (p4 * uy + sharp * nry))));
+                        hflux[2] = (T)h2f(f2h((float)((T)0.125 * (p4 * uz + sharp * nrz))));
+                    }
                     for (int q = 0; q < 7; ++q) {
-                        const T geq = g_eq(G, q, phi, ux, uy, uz);
-                        const T hi = anti_diffusion(G, q, sharp, nrx, nry, nrz);
+                        T gq;
+                        if constexpr (HALF_H) {
+                            if (q == 0) gq = (T)0.25 * phi;
+                            else if (q & 1) gq = (T)0.125 * phi + hflux[(q - 1) / 2];
+                            else gq = (T)0.125 * phi - hflux[(q - 1) / 2];
+                        } else {
+                            gq = g_eq(G, q, phi, ux, uy, uz) + anti_diffusion(G, q, sharp, nrx, nry, nrz);
+                        }
                         int xx = x + CX27[q], yy = y + CY27[q], zz = z + CZ27[q];
                         xx = wrap(xx, nx);
                         yy = wrap(yy, ny);
                         if (!jet) zz = wrap(zz, nz);
-                        st_g(q, id3(xx, yy, zz), geq + hi);
+                        st_g(q, id3(xx, yy, zz), gq);
                     }
                 }
     }
@@ -682,7 +730,9 @@ struct Sim final : Base {
 
 extern "C" {
 
-// precision: 32 = FP32, 16 = FP32 math with binary16 storage, 64 = FP64
+// precision: 32 = FP32, 16 = FP32 math with binary16 f (the reference's FP16
+// mode), 17 = FP32 math with binary16 f and binary16 phase flux (the
+// library's binary16 form), 64 = FP64
 void* pfo_create(const pf_config* cfg, int precision) {
     if (!cfg) return nullptr;
     if (cfg->velocity_set != 19 && cfg->velocity_set != 27) return nullptr;
@@ -690,6 +740,7 @@ void* pfo_create(const pf_config* cfg, int precision) {
     try {
         if (precision == 32) return static_cast<Base*>(new Sim<float, false>(*cfg));
         if (precision == 16) return static_cast<Base*>(new Sim<float, true>(*cfg));
+        if (precision == 17) return static_cast<Base*>(new Sim<float, true, true>(*cfg));
         if (precision == 64) return static_cast<Base*>(new Sim<double, false>(*cfg));
     } catch (const std::bad_alloc&) {
         return nullptr;
@@ -703,6 +754,23 @@ int pfo_step(void* h, uint64_t t0, uint32_t n) { return static_cast<Base*>(h)->s
 int pfo_get_field(void* h, int id, double* out) { return static_cast<Base*>(h)->get_field(id, out); }
 int pfo_set_field(void* h, int id, const double* in) { return static_cast<Base*>(h)->set_field(id, in); }
 int pfo_get_pop(void* h, int which, double* out) { return static_cast<Base*>(h)->get_pop(which, out); }
+
+// Derived constants of a configuration, in the oracle's precision:
+// out[0..15] = sigma, gamma, u_inf, visc_water, visc_oil, tau_water, tau_oil,
+// omco_zmin, omega_ref, sponge z_start, 1/(nz-1), 1/sponge, centre x, y, z,
+// nozzle radius^2 (src/constants.cuh:92-141,
+// src/functions/globalFunctions.cuh:104-125,260-462).
+int pfo_consts(void* h, double* out) {
+    static_cast<Base*>(h)->consts(out);
+    return PF_OK;
+}
+// Collision frequency streamCollide uses at phase phi in plane z
+// (src/lbm.cu:158-170), 1 - omega_zmax(phi) of the outflow
+// (src/BoundaryConditions.cuh:164) and the sponge ramp of plane z
+// (src/functions/deviceFunctions.cuh:94-101).
+double pfo_omega(void* h, double phi, int z) { return static_cast<Base*>(h)->omega_at(phi, z); }
+double pfo_omco_zmax(void* h, double phi) { return static_cast<Base*>(h)->omco_zmax_at(phi); }
+double pfo_sponge_ramp(void* h, int z) { return static_cast<Base*>(h)->ramp_at(z); }
 
 int pfo_num_threads(void) {
 #ifdef _OPENMP

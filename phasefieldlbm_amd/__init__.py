@@ -73,7 +73,7 @@ class Config:
     sigma: float = 0.03
     interface_width: float = 4.0
     tau_g: float = 1.0
-    phi0: float = 0.0
+    reserved0: float = 0.0  # reserved: must be 0
     device: int = -1
     zchunk: int = 0
     mlp_lo: float = 0.05

@@ -60,7 +60,7 @@ class PfConfig(ctypes.Structure):
         ("sigma", ctypes.c_float),
         ("interface_width", ctypes.c_float),
         ("tau_g", ctypes.c_float),
-        ("phi0", ctypes.c_float),
+        ("reserved0", ctypes.c_float),
         ("device", ctypes.c_int32),
         ("zchunk", ctypes.c_int32),
         ("mlp_lo", ctypes.c_float), ("mlp_hi", ctypes.c_float),

@@ -111,6 +111,12 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
     }
 }
 
+// Maximum that propagates NaN: fmax(NaN, b) returns b, which would let a
+// non-finite velocity vanish from max|u|.
+__device__ __forceinline__ double nan_max(double a, double b) {
+    return (a != a || b != b) ? __builtin_nan("") : fmax(a, b);
+}
+
 // Block reduction of NV doubles (sum, except the last, which is max when
 // LASTMAX).  256 threads = 4 wavefronts of 64.
 template <int NV, bool LASTMAX>
@@ -122,7 +128,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* out) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             const double b = __shfl_xor(a, off, 64);
-            a = (LASTMAX && k == NV - 1) ? fmax(a, b) : a + b;
+            a = (LASTMAX && k == NV - 1) ? nan_max(a, b) : a + b;
         }
         v[k] = a;
     }
@@ -135,7 +141,7 @@ __device__ __forceinline__ void block_reduce(double (&v)[NV], double* out) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double a = s[0][k];
-            for (int w = 1; w < 4; ++w) a = (LASTMAX && k == NV - 1) ? fmax(a, s[w][k]) : a + s[w][k];
+            for (int w = 1; w < 4; ++w) a = (LASTMAX && k == NV - 1) ? nan_max(a, s[w][k]) : a + s[w][k];
             out[k] = a;
         }
     }
@@ -156,7 +162,7 @@ __global__ __launch_bounds__(256) void k_diagnostics(KParams p, double* partials
         v[3] += rho * uz;
         v[4] += (double)p.fld[10][i];
         const double um = sqrt(ux * ux + uy * uy + uz * uz);
-        if (v[5] == v[5]) v[5] = (um == um) ? fmax(v[5], um) : um;  // a NaN, once seen, stays
+        v[5] = nan_max(v[5], um);  // a NaN, once seen, stays
     }
     block_reduce<6, true>(v, partials + (size_t)blockIdx.x * 6);
 }
@@ -188,7 +194,7 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* partials,
         double a = 0.0;
         for (int b = threadIdx.x; b < nblocks; b += 256) {
             const double x = partials[(size_t)b * nvals + k];
-            a = is_max ? ((x == x) ? fmax(a, x) : x) : a + x;
+            a = is_max ? nan_max(a, x) : a + x;
         }
         double v[1] = {a};
         if (is_max)
@@ -222,7 +228,7 @@ hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_init_fields(const KParams& p, bool jet, float, float, float, float, float, hipStream_t s) {
+hipError_t launch_init_fields(const KParams& p, bool jet, hipStream_t s) {
     hipLaunchKernelGGL(k_init_fields, dim3((p.N + 255) / 256), dim3(256), 0, s, p, jet ? 1 : 0);
     return hipGetLastError();
 }

@@ -117,8 +117,9 @@ hipError_t launch_phase_sum(const KParams& p, bool jet, bool local, bool half, h
 // 2 pair-symmetric with the learned curvature closure
 hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, int variant, hipStream_t s);
 hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hipStream_t s);
-hipError_t launch_init_fields(const KParams& p, bool jet, float radius, float ax, float ay, float az, float W,
-                              hipStream_t s);
+// rho = 1, the jet's nozzle plane (phi, u_z); the droplet profile is
+// uploaded by the host
+hipError_t launch_init_fields(const KParams& p, bool jet, hipStream_t s);
 hipError_t launch_init_distros(const KParams& p, int Q, bool half, hipStream_t s);
 hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, bool local, int which,
                          float* out, hipStream_t s);

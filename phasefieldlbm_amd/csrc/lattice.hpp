@@ -73,7 +73,7 @@ __host__ __device__ constexpr int first_pair() {
 // Pair-ordered first moment: for each opposite pair in increasing order with
 // c_axis != 0, add the sample of the +1 member and subtract the sample of the
 // -1 member, left to right.  This is the order of the reference's hand-written
-// sums (src/lbm.cu:72-80, src/phaseField.cuh:331-373).
+// sums (src/lbm.cu:72-80, src/phaseField.cuh:82-122 gradient, :168-210 divergence).
 template <int Q, int AXIS, int K, class V>
 __device__ __forceinline__ float pair_chain(V&& val) {
     constexpr int first = first_pair<Q, AXIS, K>();
@@ -94,7 +94,7 @@ __device__ __forceinline__ float pair_chain(V&& val) {
     return acc;
 }
 
-// Isotropic stencil sum along AXIS (src/phaseField.cuh:331-373): weight class
+// Isotropic stencil sum along AXIS (src/phaseField.cuh:82-122, :168-210): weight class
 // 1, then 2, then (D3Q27) 3.  The caller multiplies by as2.
 template <int Q, int AXIS, class V>
 __device__ __forceinline__ float iso_sum(V&& val) {

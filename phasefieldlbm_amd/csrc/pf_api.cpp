@@ -218,7 +218,7 @@ int pf_config_default(pf_config* cfg, int32_t flow_case, int32_t velocity_set) {
     cfg->closure = PF_CLOSURE_FD;
     cfg->interface_width = 4.0f;
     cfg->tau_g = 1.0f;
-    cfg->phi0 = 0.0f;
+    cfg->reserved0 = 0.0f;
     cfg->device = -1;
     cfg->seed = 0x9E3779B9u;
     cfg->noise = PF_NOISE_GAUSSIAN_XY;
@@ -267,6 +267,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     if (g.kernel != PF_KERNEL_PAIR && g.kernel != PF_KERNEL_LITERAL) return PF_ERR_CONFIG;
     if (g.derived & ~(PF_DERIVED_AVERAGE | PF_DERIVED_REYNOLDS | PF_DERIVED_VORTICITY)) return PF_ERR_CONFIG;
     if (g.tile_x != 0 && g.tile_x != pf::kTX && g.tile_x != pf::kTXNarrow) return PF_ERR_CONFIG;
+    if (g.reserved0 != 0.0f) return PF_ERR_CONFIG;
     const size_t N = (size_t)g.nx * g.ny * g.nz;
     if (N >= (size_t)1 << 30) return PF_ERR_CONFIG;  // byte offsets of one slot stay below 2^32
 
@@ -420,8 +421,7 @@ int pf_init(pf_ctx* c) {
     for (auto* d : c->der)
         if (d) PF_HIP(c, hipMemsetAsync(d, 0, c->N * sizeof(float), c->stream));
     PF_HIP(c, hipMemsetAsync(c->f, 0, c->f_elems * pop_bytes(c), c->stream));
-    PF_HIP(c, pf::launch_init_fields(c->kp, c->jet, g.radius, g.axis_x, g.axis_y, g.axis_z, g.interface_width,
-                                     c->stream));
+    PF_HIP(c, pf::launch_init_fields(c->kp, c->jet, c->stream));
     if (!c->jet) {
         // setDroplet (src/initialConditions.cu:58-83), evaluated on the host
         // with the C library's tanhf so the profile is the same everywhere.

@@ -48,7 +48,8 @@ def test_reference_defaults():
 
 
 @pytest.mark.parametrize("field,value", [("velocity_set", 15), ("flow_case", 7), ("storage", 9), ("nx", 2),
-                                         ("noise", 5), ("interface_width", 0.0), ("tile_x", 32)])
+                                         ("noise", 5), ("interface_width", 0.0), ("tile_x", 32),
+                                         ("reserved0", 1.0)])
 def test_bad_config_rejected(field, value):
     cfg = pf.presets.static_droplet(n=16)
     setattr(cfg, field, value)

@@ -146,3 +146,55 @@ def test_phase_flux_identity_fp64(case):
     # and a planted error is seen: the flux of the cell itself instead of the source
     bad = phi[z, y, x] / 8.0 + h[0][z, y, x]
     assert np.abs(g[1][z, y, x] - bad).max() > 1e-6
+
+
+@pytest.mark.parametrize("case", ["droplet", "jet"])
+def test_binary16_flux_mode(case):
+    """Precision 17 (binary16 f and phase flux, the library's binary16 form):
+    every streamed g equals phi(s)/8 +- h16_k(s) of its source cell s, where
+    h16 is h = (4 phi u + gamma phi (1 - phi) n) / 8 formed in FP32 and
+    rounded to binary16 (numpy float32 arithmetic, numpy float16 rounding),
+    and g_0 = phi/4.  Total phi stays conserved (droplet)."""
+    if case == "droplet":
+        cfg = presets.static_droplet(n=20, radius=5.0, velocity_set=D3Q27)
+        steps = 7
+    else:
+        cfg = presets.round_jet(16, 16, 32, 6, D3Q27, noise=NOISE_UNIFORM_Z)
+        steps = 9
+    o = oracle.Oracle(cfg, 17).init()
+    m0 = o.field("phi")[I].sum()
+    o.step(steps)
+    f32 = np.float32
+    phi = o.field("phi").astype(f32)
+    u = [o.field(k).astype(f32) for k in ("ux", "uy", "uz")]
+    n = [o.field(k).astype(f32) for k in ("normx", "normy", "normz")]
+    gamma = f32(o.consts()["gamma"])
+    sharp = gamma * phi * (f32(1) - phi)
+    h16 = [(f32(0.125) * (f32(4) * phi * u[a] + sharp * n[a])).astype(np.float16).astype(np.float64)
+           for a in range(3)]
+    g = o.populations("g")
+    nz, ny, nx = phi.shape
+    z0, z1 = (2, nz - 2) if case == "jet" else (1, nz - 1)
+    z, y, x = np.meshgrid(np.arange(z0, z1), np.arange(1, ny - 1), np.arange(1, nx - 1), indexing="ij")
+
+    def wrap(i, m):
+        return np.where(i < 1, m - 2, np.where(i > m - 2, 1, i))
+
+    phi64 = phi.astype(np.float64)
+    assert np.array_equal(g[0][z, y, x], (f32(0.25) * phi[z, y, x]).astype(np.float64))
+    dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    for q, c in enumerate(dirs, start=1):
+        sx, sy, sz = wrap(x - c[0], nx), wrap(y - c[1], ny), wrap(z - c[2], nz)
+        a = int(np.flatnonzero(c)[0])
+        base = f32(0.125) * phi[sz, sy, sx]
+        hv = h16[a][sz, sy, sx].astype(f32)
+        want = (base + hv if c[a] > 0 else base - hv).astype(np.float64)
+        assert np.array_equal(g[q][z, y, x], want), q
+        # the FP32 flux (precision 16's g) differs from it
+        h32 = (f32(0.125) * (f32(4) * phi * u[a] + sharp * n[a]))[sz, sy, sx]
+        assert np.abs((base + c[a] * h32) - want).max() > 0 or np.abs(h32).max() == 0
+    if case == "droplet":
+        assert abs(o.field("phi")[I].sum() - m0) / m0 < 1e-6
+    # it stays within binary16 round-off of the reference's FP16 mode
+    ref = oracle.Oracle(cfg, 16).init().step(steps)
+    assert np.abs(ref.field("phi") - phi64).max() < 2e-3

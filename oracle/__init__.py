@@ -41,6 +41,7 @@ def load():
         "pfo_get_field": (i32, [vp, i32, dp]),
         "pfo_set_field": (i32, [vp, i32, dp]),
         "pfo_get_pop": (i32, [vp, i32, dp]),
+        "pfo_set_pop": (i32, [vp, i32, dp]),
         "pfo_num_threads": (i32, []),
         "pfo_set_num_threads": (None, [i32]),
         "pfo_lattice": (i32, [i32, vp, vp, vp, vp]),
@@ -138,6 +139,14 @@ class Oracle:
         out = np.empty(self.cfg.cells, dtype=np.float64)
         assert self.lib.pfo_get_field(self.h, FIELDS.index(name), out.ctypes.data) == 0
         return out.reshape(self.cfg.shape)
+
+    def set_populations(self, which, values):
+        """Replace the post-streaming populations f (Q x shape, shifted by -w)
+        or g (7 x shape), e.g. with a GPU state."""
+        nq = self.cfg.velocity_set if which == "f" else 7
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        assert a.size == nq * self.cfg.cells
+        assert self.lib.pfo_set_pop(self.h, 0 if which == "f" else 1, a.ctypes.data) == 0
 
     def populations(self, which="f") -> np.ndarray:
         nq = self.cfg.velocity_set if which == "f" else 7

@@ -277,6 +277,7 @@ struct Base {
     virtual int get_field(int id, double* out) = 0;
     virtual int set_field(int id, const double* in) = 0;
     virtual int get_pop(int which, double* out) = 0;
+    virtual int set_pop(int which, const double* in) = 0;
 };
 
 // HALF: f stored in binary16.  HALF_H: the phase populations are formed from
@@ -714,6 +715,18 @@ struct Sim final : Base {
         for (size_t i = 0; i < N; ++i) fld[id][i] = (T)in[i];
         return PF_OK;
     }
+    // Replace the post-streaming populations (the state the next step reads),
+    // e.g. with a GPU state, to take one step from it.
+    int set_pop(int which, const double* in) override {
+        if (which == 0) {
+            for (size_t q = 0; q < (size_t)L.Q; ++q)
+                for (size_t i = 0; i < N; ++i) st_f((int)q, i, (T)in[q * N + i]);
+        } else {
+            for (size_t q = 0; q < 7; ++q)
+                for (size_t i = 0; i < N; ++i) st_g((int)q, i, (T)in[q * N + i]);
+        }
+        return PF_OK;
+    }
     int get_pop(int which, double* out) override {
         if (which == 0) {
             for (size_t q = 0; q < (size_t)L.Q; ++q)
@@ -754,6 +767,7 @@ int pfo_step(void* h, uint64_t t0, uint32_t n) { return static_cast<Base*>(h)->s
 int pfo_get_field(void* h, int id, double* out) { return static_cast<Base*>(h)->get_field(id, out); }
 int pfo_set_field(void* h, int id, const double* in) { return static_cast<Base*>(h)->set_field(id, in); }
 int pfo_get_pop(void* h, int which, double* out) { return static_cast<Base*>(h)->get_pop(which, out); }
+int pfo_set_pop(void* h, int which, const double* in) { return static_cast<Base*>(h)->set_pop(which, in); }
 
 // Derived constants of a configuration, in the oracle's precision:
 // out[0..15] = sigma, gamma, u_inf, visc_water, visc_oil, tau_water, tau_oil,

@@ -1,11 +1,16 @@
 """Accuracy of binary16 population storage against the FP64 oracle.
 
-Workload: 128x128x512 D3Q27 round jet (D = 16, u = 0.05, Re = 5000,
-We = 500, uniform +-5 % u_z inflow noise).  The FP64 reference fields come
-from tests/fp64_snapshots.py (about 5 h of CPU for 5000 steps), so the tests
-run only when PF_FP64_SNAPSHOTS names a directory holding them; otherwise they
-are skipped.  For every snapshot step the library runs with FP16 and with
-FP32 storage, and the error over interior cells is
+Two workloads, both D3Q27 round jets (u = 0.05, Re = 5000, We = 500,
+uniform +-5 % u_z inflow noise):
+  * 40x40x128, D = 8: the committed FP64 fixture
+    tests/golden/fp64_jet_d3q27_40x40x128.npz (written by
+    tests/golden/make_fp64_jet.py, a few minutes of CPU) at 500, 1000 and
+    2000 steps; these tests always run on the GPU;
+  * 128x128x512, D = 16, the benchmark's accuracy case: FP64 snapshots from
+    tests/fp64_snapshots.py (about 5 h of CPU for 5000 steps), run only when
+    PF_FP64_SNAPSHOTS names a directory holding them.
+For every snapshot step the library runs with FP16 and with FP32 storage,
+and the error over interior cells is
 
     rel_l2(u)   = ||u - u64||_2 / ||u64||_2       (vector field)
     rel_l2(phi) = ||phi - phi64||_2 / ||phi64||_2
@@ -28,6 +33,7 @@ import numpy as np
 import pytest
 
 SNAP = os.environ.get("PF_FP64_SNAPSHOTS", "")
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fp64_jet_d3q27_40x40x128.npz")
 
 
 def _steps(d):
@@ -116,3 +122,74 @@ def test_fp16_storage_against_fp64_at_5000_steps(report):
     if 5000 not in report:
         pytest.skip("no snapshot at step 5000")
     _check(report[5000], 5000)
+
+
+# ---------------------------------------------------------------------------
+# The reduced jet of the committed fixture
+# ---------------------------------------------------------------------------
+
+def _reduced_config(storage=0):
+    from phasefieldlbm_amd import D3Q27, presets
+    return presets.round_jet(40, 40, 128, 8, D3Q27, storage=storage)
+
+
+def _reduced_metrics(got, fix, s):
+    """Errors of interior fields `got` (ux, uy, uz, phi; ghost shell included
+    in the arrays) against the fixture at step s."""
+    I = (slice(1, -1),) * 3
+    u = np.stack([got[k][I] for k in ("ux", "uy", "uz")]).astype(np.float64)
+    u64 = np.stack([fix[f"{k}_{s}"] for k in ("ux", "uy", "uz")]).astype(np.float64)
+    ph, ph64 = got["phi"][I].astype(np.float64), fix[f"phi_{s}"].astype(np.float64)
+    return {
+        "rel_l2_u": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+        "rel_linf_u": float(np.abs(u - u64).max() / np.abs(u64).max()),
+        "rel_l2_phi": float(np.linalg.norm(ph - ph64) / np.linalg.norm(ph64)),
+        "rel_linf_phi": float(np.abs(ph - ph64).max() / np.abs(ph64).max()),
+        "phase_mass_drift": float((ph.sum() - float(fix[f"sum_phi_{s}"])) / float(fix[f"sum_phi_{s}"])),
+    }
+
+
+@pytest.mark.slow
+def test_reduced_fixture_matches_oracle():
+    """The fixture is the oracle's: the FP32 oracle follows it to FP32 round-off
+    growth after 500 steps (no GPU)."""
+    import oracle
+    fix = np.load(FIXTURE)
+    assert list(fix["steps"]) == [500, 1000, 2000]
+    o = oracle.Oracle(_reduced_config(), 32).init().step(500)
+    got = {k: o.field(k) for k in ("ux", "uy", "uz", "phi")}
+    m = _reduced_metrics(got, fix, 500)
+    print(m)
+    assert m["rel_l2_u"] < 1e-4 and m["rel_l2_phi"] < 1e-4
+    # the jet has developed: phi = 1 fluid well inside the domain
+    assert fix["phi_2000"][40:].max() > 0.9 and np.abs(fix["uz_2000"]).max() > 0.04
+
+
+@pytest.fixture(scope="module")
+def reduced_report():
+    import phasefieldlbm_amd as pf
+    fix = np.load(FIXTURE)
+    steps = [int(s) for s in fix["steps"]]
+    rep = {}
+    for name, storage in (("fp16", pf.STORE_FP16), ("fp32", pf.STORE_FP32)):
+        with pf.Simulation(_reduced_config(storage)) as sim:
+            sim.init()
+            done = 0
+            for s in steps:
+                sim.step(s - done, save=True)
+                done = s
+                got = {k: sim.field(k) for k in ("ux", "uy", "uz", "phi")}
+                rep.setdefault(s, {})[name] = _reduced_metrics(got, fix, s)
+    print(json.dumps({"fp16_accuracy_vs_fp64_40x40x128": rep}), flush=True)
+    return rep
+
+
+@pytest.mark.gpu
+def test_fp16_reduced_jet_against_fp64(reduced_report):
+    """Binary16 storage on the reduced jet against the committed FP64 fixture:
+    within max(1e-3, 3 x the FP32 floor) on u and phi up to 1000 steps, and
+    FP32 storage itself within 1e-3."""
+    for s in (500, 1000):
+        r = reduced_report[s]
+        assert r["fp32"]["rel_l2_u"] < 1e-3 and r["fp32"]["rel_l2_phi"] < 1e-3, (s, r)
+        _check(r, s)

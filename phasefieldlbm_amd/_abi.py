@@ -23,6 +23,7 @@ STEP_SAVE_LAST = 1
 STEP_GRAPH = 2
 OUT_BIN, OUT_VTI, OUT_VTR = 1, 2, 4
 MLP_PARAMS = 6017
+MLP_PREDICT_MAX = 8192  # PF_MLP_PREDICT_MAX
 
 # host::FieldID order (src/fileIO/fields.cuh:33-68): the timestep's fields,
 # then the derived fields (enabled through Config.derived)
@@ -101,6 +102,7 @@ SIGNATURES = {
     "pf_mlp_gradients": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                         ctypes.POINTER(ctypes.c_double)]),
     "pf_mlp_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "pf_mlp_predict_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "pf_mfma_check": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pf_write_fields": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint64, ctypes.c_void_p,
                                        ctypes.c_int32, ctypes.c_int32]),

@@ -19,6 +19,7 @@
 #include "../../include/phasefieldlbm.h"
 #include "kparams.hpp"
 #include "pf_io.hpp"
+#include "pf_mlp.hpp"
 
 struct pf_ctx {
     pf_config cfg{};
@@ -756,19 +757,6 @@ int pf_time_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, float out_ms[4]) {
 // ---------------------------------------------------------------------------
 // learned closure: training on the device
 // ---------------------------------------------------------------------------
-struct pf_mlp {
-    int dev = 0;
-    hipStream_t stream = nullptr;
-    float *master = nullptr, *m1 = nullptr, *m2 = nullptr, *bias = nullptr, *parts = nullptr, *gsum = nullptr;
-    uint16_t* w16 = nullptr;
-    float* xs = nullptr;
-    float* ts = nullptr;
-    int* idx = nullptr;
-    uint8_t* sym = nullptr;
-    double* loss = nullptr;
-    size_t n = 0, cap_parts = 0;
-    int step = 0;
-};
 
 extern "C++" {
 namespace {
@@ -842,6 +830,8 @@ int pf_mlp_create(int32_t device, pf_mlp** out) {
     ok = ok && hipMalloc((void**)&m->loss, sizeof(double)) == hipSuccess;
     const std::vector<uint8_t> sym = cube_symmetries();
     ok = ok && hipMalloc((void**)&m->sym, sym.size()) == hipSuccess;
+    // the fixed staging buffer of pf_mlp_predict_noalloc on this device
+    ok = ok && pf::mlp_scratch_setup(device) == PF_OK;
     ok = ok && hipMemcpy(m->sym, sym.data(), sym.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         pf_mlp_destroy(m);

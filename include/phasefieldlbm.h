@@ -271,6 +271,11 @@ int pf_time_steps(pf_ctx *ctx, uint64_t t0, uint32_t nsteps, float out_ms[4]);
  * / time.  No counterpart in the reference; used to state the measured peak
  * beside the roofline and to calibrate memory-traffic counters. */
 int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double *gbs);
+/* The same measurement for three streaming kernels over `bytes`-byte
+ * arrays: gbs[0] read-only, gbs[1] write-only, gbs[2] copy (read + write
+ * bytes, the value pf_copy_bandwidth returns).  No counterpart in the
+ * reference. */
+int pf_stream_bandwidth(int32_t device, size_t bytes, uint32_t iters, double gbs[3]);
 
 /*
  * Output.  Formats are a bit mask: PF_OUT_BIN writes <dir>/<name><step6>.bin,

@@ -23,7 +23,8 @@ from ._abi import (CLOSURE_FD, CLOSURE_MLP, D3Q19, D3Q27, DERIVED_AVERAGE, DERIV
 
 __all__ = ["Config", "Simulation", "PfError", "presets", "FIELDS", "D3Q19", "D3Q27", "JET", "DROPLET",
            "STORE_FP32", "STORE_FP16", "NOISE_GAUSSIAN_XY", "NOISE_UNIFORM_Z", "NOISE_NONE",
-           "CLOSURE_FD", "CLOSURE_MLP", "KERNEL_PAIR", "KERNEL_LITERAL", "library_path", "copy_bandwidth", "MLP",
+           "CLOSURE_FD", "CLOSURE_MLP", "KERNEL_PAIR", "KERNEL_LITERAL", "library_path", "copy_bandwidth",
+           "stream_bandwidth", "MLP",
            "split_params", "mfma_check", "vtk", "DERIVED_FIELDS", "DERIVED_AVERAGE", "DERIVED_REYNOLDS",
            "DERIVED_VORTICITY"]
 
@@ -39,6 +40,16 @@ def copy_bandwidth(device: int = 0, nbytes: int = 2 << 30, iters: int = 20) -> f
     if rc:
         raise PfError(rc, "pf_copy_bandwidth")
     return g.value
+
+
+def stream_bandwidth(device: int = 0, nbytes: int = 4 << 30, iters: int = 10) -> dict:
+    """Measured HBM bandwidth (GB/s) of read-only, write-only and copy
+    streaming kernels over `nbytes`-byte arrays."""
+    out = np.zeros(3, dtype=np.float64)
+    rc = _abi.load().pf_stream_bandwidth(device, nbytes, iters, out.ctypes.data)
+    if rc:
+        raise PfError(rc, "pf_stream_bandwidth")
+    return {"read_gbs": float(out[0]), "write_gbs": float(out[1]), "copy_gbs": float(out[2])}
 
 
 class PfError(RuntimeError):

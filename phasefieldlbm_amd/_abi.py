@@ -92,6 +92,7 @@ SIGNATURES = {
     "pf_device_bytes": (ctypes.c_size_t, [ctypes.c_void_p]),
     "pf_copy_bandwidth": (ctypes.c_int, [ctypes.c_int32, ctypes.c_size_t, ctypes.c_uint32,
                                          ctypes.POINTER(ctypes.c_double)]),
+    "pf_stream_bandwidth": (ctypes.c_int, [ctypes.c_int32, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_void_p]),
     "pf_mlp_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     "pf_mlp_destroy": (None, [ctypes.c_void_p]),
     "pf_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

@@ -205,10 +205,14 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* partials,
     }
 }
 
-// Streaming copy used to measure the achievable HBM bandwidth (and to
-// calibrate memory-traffic counters against a known byte count).
+// Streaming kernels that measure the achievable HBM bandwidth (and
+// calibrate memory-traffic counters against a known byte count): copy,
+// read-only and write-only, four 16-byte accesses per thread, one pass over
+// the array (a grid of n/1024 workgroups of 256 threads).  Measured on
+// MI355X, 4 GiB: copy 5.52-5.57 TB/s, read-only 6.15-6.17, write-only
+// 5.82; a grid-stride copy on 1024-65536 workgroups, 2-16 accesses per
+// thread and non-temporal loads and stores all stayed at 4.6-5.54 TB/s.
 __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
-    // four independent 16-byte loads in flight per thread before the stores
     const size_t stride = (size_t)gridDim.x * 1024u;
     size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x;
     for (; i + 768 < n4; i += stride) {
@@ -221,10 +225,52 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
     for (; i < n4; i += 256) dst[i] = src[i];
 }
 
+// Reads every element; the sum is stored only if it equals a value a zeroed
+// array cannot produce, so the loads stay live.
+__global__ __launch_bounds__(256) void k_read_only(const float4* __restrict__ src, size_t n4, float* sink) {
+    const size_t stride = (size_t)gridDim.x * 1024u;
+    size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x;
+    float acc = 0.0f;
+    for (; i + 768 < n4; i += stride) {
+        const float4 a = src[i], b = src[i + 256], c = src[i + 512], d = src[i + 768];
+        acc += ((a.x + a.y) + (a.z + a.w)) + ((b.x + b.y) + (b.z + b.w)) + ((c.x + c.y) + (c.z + c.w)) +
+               ((d.x + d.y) + (d.z + d.w));
+    }
+    for (; i < n4; i += 256) acc += src[i].x;
+    if (acc == 1234.5f) sink[0] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_write_only(float4* __restrict__ dst, size_t n4) {
+    const size_t stride = (size_t)gridDim.x * 1024u;
+    size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x;
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (; i + 768 < n4; i += stride) {
+        dst[i] = z;
+        dst[i + 256] = z;
+        dst[i + 512] = z;
+        dst[i + 768] = z;
+    }
+    for (; i < n4; i += 256) dst[i] = z;
+}
+
 // ---------------------------------------------------------------------------
-hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s) {
+static dim3 stream_grid(size_t n4) {
     const size_t blocks = (n4 + 1023) / 1024;
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256), 0, s, src, dst, n4);
+    return dim3((unsigned)(blocks < 0x7fffffffu ? blocks : 0x7fffffffu));
+}
+
+hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy, stream_grid(n4), dim3(256), 0, s, src, dst, n4);
+    return hipGetLastError();
+}
+
+hipError_t launch_read_only(const float4* src, size_t n4, float* sink, hipStream_t s) {
+    hipLaunchKernelGGL(k_read_only, stream_grid(n4), dim3(256), 0, s, src, n4, sink);
+    return hipGetLastError();
+}
+
+hipError_t launch_write_only(float4* dst, size_t n4, hipStream_t s) {
+    hipLaunchKernelGGL(k_write_only, stream_grid(n4), dim3(256), 0, s, dst, n4);
     return hipGetLastError();
 }
 

@@ -128,6 +128,8 @@ hipError_t launch_phase_mass(const KParams& p, bool jet, bool local, bool half, 
                              hipStream_t s);
 hipError_t launch_reduce_partials(const double* partials, int nblocks, int nvals, double* out, hipStream_t s);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n4, hipStream_t s);
+hipError_t launch_read_only(const float4* src, size_t n4, float* sink, hipStream_t s);
+hipError_t launch_write_only(float4* dst, size_t n4, hipStream_t s);
 
 // Derived fields (src/derivedFields/): avg[0..3] running means of phi, ux,
 // uy, uz; avg[4..9] of ux*ux, uy*uy, uz*uz, ux*uy, ux*uz, uy*uz; vort[0..3]

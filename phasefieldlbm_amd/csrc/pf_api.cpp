@@ -1001,16 +1001,21 @@ int pf_mfma_check(int32_t device, const float* a, const float* b, float* c) {
     return rc;
 }
 
-int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double* gbs) {
-    if (!gbs || bytes < 1024 || iters == 0) return PF_ERR_CONFIG;
+// Times `iters` launches of each streaming kernel over two arrays of
+// `bytes` bytes with HIP events: out[0] read-only, out[1] write-only, out[2]
+// copy (read + write bytes counted), in GB/s.
+static int stream_bandwidth(int32_t device, size_t bytes, uint32_t iters, double out[3]) {
+    if (!out || bytes < 1024 || iters == 0) return PF_ERR_CONFIG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return PF_ERR_DEVICE;
     if (hipSetDevice(device) != hipSuccess) return PF_ERR_DEVICE;
     const size_t n4 = bytes / 16;
     float4 *a = nullptr, *b = nullptr;
+    float* sink = nullptr;
     if (hipMalloc((void**)&a, n4 * 16) != hipSuccess) return PF_ERR_OOM;
-    if (hipMalloc((void**)&b, n4 * 16) != hipSuccess) {
+    if (hipMalloc((void**)&b, n4 * 16) != hipSuccess || hipMalloc((void**)&sink, 16) != hipSuccess) {
         (void)hipFree(a);
+        (void)hipFree(b);
         return PF_ERR_OOM;
     }
     hipStream_t s;
@@ -1020,21 +1025,43 @@ int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double* gbs)
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = PF_ERR_DEVICE;
     if (!rc) {
         (void)hipMemsetAsync(a, 0, n4 * 16, s);
-        (void)pf::launch_copy(a, b, n4, s);  // warm-up
-        (void)hipEventRecord(e0, s);
-        for (uint32_t i = 0; i < iters; ++i) (void)pf::launch_copy(a, b, n4, s);
-        (void)hipEventRecord(e1, s);
-        if (hipEventSynchronize(e1) != hipSuccess) rc = PF_ERR_LAUNCH;
-        float ms = 0.0f;
-        if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PF_ERR_LAUNCH;
-        if (!rc) *gbs = 2.0 * (double)(n4 * 16) * iters / (ms * 1e-3) / 1e9;
+        const double moved[3] = {(double)(n4 * 16), (double)(n4 * 16), 2.0 * (double)(n4 * 16)};
+        for (int k = 0; k < 3 && !rc; ++k) {
+            auto launch = [&]() {
+                if (k == 0) return pf::launch_read_only(a, n4, sink, s);
+                if (k == 1) return pf::launch_write_only(b, n4, s);
+                return pf::launch_copy(a, b, n4, s);
+            };
+            if (launch() != hipSuccess) rc = PF_ERR_LAUNCH;  // warm-up
+            (void)hipEventRecord(e0, s);
+            for (uint32_t i = 0; i < iters && !rc; ++i)
+                if (launch() != hipSuccess) rc = PF_ERR_LAUNCH;
+            (void)hipEventRecord(e1, s);
+            if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = PF_ERR_LAUNCH;
+            float ms = 0.0f;
+            if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PF_ERR_LAUNCH;
+            if (!rc) out[k] = moved[k] * iters / (ms * 1e-3) / 1e9;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(s);
     }
     (void)hipFree(a);
     (void)hipFree(b);
+    (void)hipFree(sink);
     return rc;
+}
+
+int pf_copy_bandwidth(int32_t device, size_t bytes, uint32_t iters, double* gbs) {
+    if (!gbs) return PF_ERR_CONFIG;
+    double out[3] = {0, 0, 0};
+    const int rc = stream_bandwidth(device, bytes, iters, out);
+    if (!rc) *gbs = out[2];
+    return rc;
+}
+
+int pf_stream_bandwidth(int32_t device, size_t bytes, uint32_t iters, double gbs[3]) {
+    return stream_bandwidth(device, bytes, iters, gbs);
 }
 
 }  // extern "C"

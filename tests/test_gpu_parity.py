@@ -458,6 +458,10 @@ def test_slot_padding_invariance():
 def test_copy_bandwidth_probe():
     gbs = pf.copy_bandwidth(0, 1 << 30, 5)
     assert 500.0 < gbs < 12000.0
+    bw = pf.stream_bandwidth(0, 1 << 30, 5)
+    print(f"\n[bandwidth] {bw}", flush=True)
+    for k in ("read_gbs", "write_gbs", "copy_gbs"):
+        assert 500.0 < bw[k] < 8000.0, (k, bw)
 
 
 def test_state_errors():

@@ -36,8 +36,13 @@ enum pf_flow_case { PF_JET = 0, PF_DROPLET = 1 };
  * the phase flux (the 3-vector from which the D3Q7 populations are rebuilt)
  * in binary16 (round to nearest even) with FP32 arithmetic, the role of
  * ENABLE_FP16 (src/cuda/precision.cuh:29-42), which in the reference stores
- * f in FP16 and g in FP32. */
-enum pf_storage { PF_STORE_FP32 = 0, PF_STORE_FP16 = 1 };
+ * f in FP16 and g in FP32.  FP16_NE stores in binary16 2^12 times the
+ * population's departure from the second-order equilibrium of the cell that
+ * wrote it, w (A + b + b^2/2) with b = c.B, (A, B) = (rho (1 - 1.5|u|^2) - 1,
+ * 3 rho u) of that cell, kept per cell in FP32 (16 B more per cell and step
+ * written and read); the phase flux as in FP16.  Pair collision only
+ * (PF_KERNEL_PAIR); no counterpart in the reference. */
+enum pf_storage { PF_STORE_FP32 = 0, PF_STORE_FP16 = 1, PF_STORE_FP16_NE = 2 };
 
 /* Inflow perturbation.  GAUSSIAN_XY is the reference's Box-Muller noise with
  * sigma = 0.08*u_inf on u_x, u_y (src/BoundaryConditions.cuh:60-69).

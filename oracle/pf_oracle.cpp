@@ -280,13 +280,48 @@ struct Base {
     virtual int set_pop(int which, const double* in) = 0;
 };
 
+// Shift of the library's binary16 non-equilibrium form: M = (A, B) =
+// (rho (1 - 1.5|u|^2) - 1, 3 rho u), rounded to binary16, of the cell that
+// writes a population,
+// S_q(M) = w_q (A + b (1 + b/2)), b = c_q . B; the population is stored as
+// K (f~ - S_q(M)) in binary16 with K = 2^12 and read back as
+// stored / K + S_q(M).  Evaluated in the library's operation order.
+struct NeMom {
+    float A, Bx, By, Bz;
+};
+// M is kept rounded to binary16, as the library stores it
+inline NeMom ne_mom(float rho, float ux, float uy, float uz) {
+    const float v = 1.0f - 1.5f * (ux * ux + uy * uy + uz * uz);
+    const float r3 = 3.0f * rho;
+    auto r16 = [](float a) { return h2f(f2h(a)); };
+    return {r16(rho * v - 1.0f), r16(r3 * ux), r16(r3 * uy), r16(r3 * uz)};
+}
+inline float ne_shift(const Lat<float>& L, int q, const NeMom& m) {
+    const float w = L.w[q];
+    if (q == 0) return w * m.A;
+    float b = 0.0f;
+    bool any = false;
+    const float comp[3] = {m.Bx, m.By, m.Bz};
+    const int c[3] = {CX27[q], CY27[q], CZ27[q]};
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] == 0) continue;
+        b = any ? (c[a] > 0 ? b + comp[a] : b - comp[a]) : (c[a] > 0 ? comp[a] : -comp[a]);
+        any = true;
+    }
+    return w * (m.A + b * (1.0f + 0.5f * b));
+}
+constexpr float kNeK = 4096.0f;
+
 // HALF: f stored in binary16.  HALF_H: the phase populations are formed from
-// a binary16 phase flux, as the library's binary16 form does.
-template <class T, bool HALF, bool HALF_H = false>
+// a binary16 phase flux, as the library's binary16 form does.  NE: f stored
+// as the library's binary16 non-equilibrium form does (kept here as the FP32
+// value that form reads back).
+template <class T, bool HALF, bool HALF_H = false, bool NE = false>
 struct Sim final : Base {
     using S = typename std::conditional<HALF, uint16_t, T>::type;
     pf_config c;
     Lat<T> L, G;
+    Lat<float> Lf;  // the lattice in FP32, for the non-equilibrium shift
     Consts<T> k;
     int nx, ny, nz;
     size_t N;
@@ -300,6 +335,7 @@ struct Sim final : Base {
         N = (size_t)nx * ny * nz;
         L = make_lat<T>(c.velocity_set);
         G = make_lat<T>(7);
+        Lf = make_lat<float>(c.velocity_set);
         jet = c.flow_case == PF_JET;
         for (auto& v : fld) v.assign(N, (T)0);
         f.assign((size_t)L.Q * N, S{});
@@ -351,6 +387,17 @@ struct Sim final : Base {
         if constexpr (HALF) f[(size_t)q * N + i] = f2h((float)v);
         else f[(size_t)q * N + i] = v;
     }
+    // store as the non-equilibrium form does, written by a cell with moments m
+    void st_f_ne(int q, size_t i, T v, const NeMom& m) {
+        if constexpr (NE) {
+            const float sh = ne_shift(Lf, q, m);
+            const float stored = h2f(f2h(kNeK * ((float)v - sh)));
+            f[(size_t)q * N + i] = (T)(sh + stored / kNeK);
+        } else {
+            (void)m;
+            st_f(q, i, v);
+        }
+    }
     T ld_g(int q, size_t i) const { return g[(size_t)q * N + i]; }
     void st_g(int q, size_t i, T v) { g[(size_t)q * N + i] = v; }
 
@@ -396,8 +443,21 @@ struct Sim final : Base {
                     const size_t i = id3(x, y, z);
                     const T ux = F(PF_FIELD_UX, i), uy = F(PF_FIELD_UY, i), uz = F(PF_FIELD_UZ, i);
                     const T uu = (T)1.5 * (ux * ux + uy * uy + uz * uz);
-                    for (int q = 0; q < L.Q; ++q)
-                        st_f(q, i, f_eq(L, q, F(PF_FIELD_RHO, i), uu, cu_q(L, q, ux, uy, uz)));
+                    const bool interior = !guard(x, y, z);
+                    for (int q = 0; q < L.Q; ++q) {
+                        const T feq = f_eq(L, q, F(PF_FIELD_RHO, i), uu, cu_q(L, q, ux, uy, uz));
+                        if (NE && interior) {
+                            // held for the first step as written by x - c_q
+                            const int wx = wrap(x - CX27[q], nx), wy = wrap(y - CY27[q], ny);
+                            const int wz = jet ? z - CZ27[q] : wrap(z - CZ27[q], nz);
+                            const size_t w = id3(wx, wy, wz);
+                            st_f_ne(q, i, feq,
+                                    ne_mom((float)F(PF_FIELD_RHO, w), (float)F(PF_FIELD_UX, w), (float)F(PF_FIELD_UY, w),
+                                           (float)F(PF_FIELD_UZ, w)));
+                        } else {
+                            st_f(q, i, feq);
+                        }
+                    }
                     for (int q = 0; q < 7; ++q) st_g(q, i, g_eq(G, q, F(PF_FIELD_PHI, i), ux, uy, uz));
                 }
         return PF_OK;
@@ -559,6 +619,7 @@ struct Sim final : Base {
                     const T omega = omega_cell(phi, z);
                     const T omco = (T)1 - omega;
                     const T uu = (T)1.5 * (ux * ux + uy * uy + uz * uz);
+                    const NeMom mw = ne_mom((float)rho, (float)ux, (float)uy, (float)uz);
                     for (int q = 0; q < L.Q; ++q) {
                         const T cu = cu_q(L, q, ux, uy, uz);
                         const T feq = f_eq(L, q, rho, uu, cu);
@@ -568,7 +629,7 @@ struct Sim final : Base {
                         xx = wrap(xx, nx);
                         yy = wrap(yy, ny);
                         if (!jet) zz = wrap(zz, nz);
-                        st_f(q, id3(xx, yy, zz), feq + omco * fneq + (T)0.5 * frc);
+                        st_f_ne(q, id3(xx, yy, zz), feq + omco * fneq + (T)0.5 * frc, mw);
                     }
                     const T nrx = F(PF_FIELD_NORMX, i), nry = F(PF_FIELD_NORMY, i), nrz = F(PF_FIELD_NORMZ, i);
                     const T sharp = k.gamma * phi * ((T)1 - phi);
@@ -658,7 +719,7 @@ struct Sim final : Base {
                     const T fneq = f_neq(L, q, F(PF_FIELD_PXX, fn), F(PF_FIELD_PYY, fn), F(PF_FIELD_PZZ, fn),
                                          F(PF_FIELD_PXY, fn), F(PF_FIELD_PXZ, fn), F(PF_FIELD_PYZ, fn),
                                          F(PF_FIELD_UX, fn), F(PF_FIELD_UY, fn), F(PF_FIELD_UZ, fn));
-                    st_f(q, fn, feq + k.omco_zmin * fneq);
+                    st_f_ne(q, fn, feq + k.omco_zmin * fneq, ne_mom(1.0f, (float)ux, (float)uy, (float)uz));
                 }
                 st_g(5, id3(x, y, 1), G.w[5] * phi * ((T)1 + G.as2 * uz));
             }
@@ -684,7 +745,8 @@ struct Sim final : Base {
                                          F(PF_FIELD_UX, fn), F(PF_FIELD_UY, fn), F(PF_FIELD_UZ, fn));
                     const T frc = force_q(L, q, cu, ux, uy, uz, F(PF_FIELD_FSX, fn), F(PF_FIELD_FSY, fn),
                                           F(PF_FIELD_FSZ, fn));
-                    st_f(q, fn, feq + omco_zmax(phi) * fneq + (T)0.5 * frc);
+                    st_f_ne(q, fn, feq + omco_zmax(phi) * fneq + (T)0.5 * frc,
+                            ne_mom((float)rho, (float)ux, (float)uy, (float)uz));
                 }
                 st_g(6, im, G.w[6] * phi * ((T)1 - G.as2 * uz));
             }
@@ -745,7 +807,8 @@ extern "C" {
 
 // precision: 32 = FP32, 16 = FP32 math with binary16 f (the reference's FP16
 // mode), 17 = FP32 math with binary16 f and binary16 phase flux (the
-// library's binary16 form), 64 = FP64
+// library's binary16 form), 18 = the library's binary16 non-equilibrium
+// form (f as it reads it back, binary16 phase flux), 64 = FP64
 void* pfo_create(const pf_config* cfg, int precision) {
     if (!cfg) return nullptr;
     if (cfg->velocity_set != 19 && cfg->velocity_set != 27) return nullptr;
@@ -754,6 +817,7 @@ void* pfo_create(const pf_config* cfg, int precision) {
         if (precision == 32) return static_cast<Base*>(new Sim<float, false>(*cfg));
         if (precision == 16) return static_cast<Base*>(new Sim<float, true>(*cfg));
         if (precision == 17) return static_cast<Base*>(new Sim<float, true, true>(*cfg));
+        if (precision == 18) return static_cast<Base*>(new Sim<float, false, true, true>(*cfg));
         if (precision == 64) return static_cast<Base*>(new Sim<double, false>(*cfg));
     } catch (const std::bad_alloc&) {
         return nullptr;

@@ -13,6 +13,7 @@
 
 #include "kparams.hpp"
 #include "lattice.hpp"
+#include "ne.hpp"
 #include "phase_flux.hpp"
 
 namespace pf {
@@ -55,7 +56,31 @@ __global__ __launch_bounds__(256) void k_init_fields(KParams p, int jet) {
     }
 }
 
-template <int Q, bool HALF>
+// Cell indices of the 3x3x3 neighbourhood with the ghost-shell wrap of the
+// timestep (x, y always; z for the droplet), as make_nbr.
+template <bool WRAPZ>
+__device__ __forceinline__ void cell_nbr(const KParams& p, int x, int y, int z, uint32_t (&cx)[3], uint32_t (&cy)[3],
+                                         uint32_t (&cz)[3]) {
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int xm = (x == 1) ? nx - 2 : x - 1, xp = (x == nx - 2) ? 1 : x + 1;
+    const int ym = (y == 1) ? ny - 2 : y - 1, yp = (y == ny - 2) ? 1 : y + 1;
+    int zm = z - 1, zp = z + 1;
+    if constexpr (WRAPZ) {
+        zm = (z == 1) ? nz - 2 : z - 1;
+        zp = (z == nz - 2) ? 1 : z + 1;
+    }
+    const uint32_t row = (uint32_t)nx, plane = (uint32_t)nx * (uint32_t)ny;
+    cx[0] = (uint32_t)xm; cx[1] = (uint32_t)x; cx[2] = (uint32_t)xp;
+    cy[0] = row * (uint32_t)ym; cy[1] = row * (uint32_t)y; cy[2] = row * (uint32_t)yp;
+    cz[0] = plane * (uint32_t)zm; cz[1] = plane * (uint32_t)z; cz[2] = plane * (uint32_t)zp;
+}
+
+// Non-equilibrium form: the population of direction q that cell (x, y, z)
+// holds for the first (even) step counts as written by x - c_q, so it is
+// stored relative to that cell's shift; every cell's M goes into both
+// M buffers.  Ghost cells hold zeros (no step reads them before the boundary
+// kernel rewrites them).
+template <int Q, bool HALF, bool NE, bool WRAPZ>
 __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= p.N) return;
@@ -64,6 +89,25 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
     const int x = (int)(i % (uint32_t)p.nx);
     const int y = (int)((i / (uint32_t)p.nx) % (uint32_t)p.ny);
     const int z = (int)(i / ((uint32_t)p.nx * (uint32_t)p.ny));
+    if constexpr (NE) {
+        const NeM me = ne_mom(rho, ux, uy, uz);
+        p.mom_cur[i] = me.packed;
+        const_cast<uint2*>(p.mom_prev)[i] = me.packed;
+        const bool interior = x >= 1 && x <= p.nx - 2 && y >= 1 && y <= p.ny - 2 && z >= 1 && z <= p.nz - 2;
+        uint32_t cx[3] = {0, 0, 0}, cy[3] = {0, 0, 0}, cz[3] = {0, 0, 0};
+        if (interior) cell_nbr<WRAPZ>(p, x, y, z, cx, cy, cz);
+        static_for<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            float v = 0.0f;
+            if (interior) {
+                const uint32_t w = cx[1 - kCX[q]] + cy[1 - kCY[q]] + cz[1 - kCZ[q]];
+                const float4 mw = ne_mom(p.fld[0][w], p.fld[1][w], p.fld[2][w], p.fld[3][w]).m;
+                v = p.ne_k * (f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)) - ne_shift<Q, q>(mw));
+            }
+            st_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, p.nx, q, x, y, z), v);
+        });
+        return;
+    }
     static_for<Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         st_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, p.nx, q, x, y, z), f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
@@ -72,7 +116,7 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
 
 // which = 0: f (Q slots, stored shift -w kept); which = 1: g (7 slots, the
 // populations the next phase sum adds)
-template <int Q, bool JET, bool HALF, int PAR, bool LOCAL>
+template <int Q, bool JET, bool HALF, int PAR, bool LOCAL, bool NE = false>
 __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= p.N) return;
@@ -93,6 +137,29 @@ __global__ __launch_bounds__(256) void k_gather(KParams p, int which, float* out
         float g[7] = {0, 0, 0, 0, 0, 0, 0};
         if (interior) arriving_g<JET, LOCAL, HALF>(p, x, y, z, g);
         for (int q = 0; q < 7; ++q) out[(size_t)q * p.N + i] = g[q];
+        return;
+    }
+    if constexpr (NE) {
+        // stored value plus the shift of the cell that wrote it, x - c_q (ne.hpp)
+        uint32_t cx[3] = {0, 0, 0}, cy[3] = {0, 0, 0}, cz[3] = {0, 0, 0};
+        if (interior) cell_nbr<!JET>(p, x, y, z, cx, cy, cz);
+        static_for<Q>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            float v = 0.0f;
+            if (interior) {
+                int sx = x, sy = y, sz = z, slot = q;
+                if constexpr (PAR == 1) {
+                    slot = opp(q);
+                    sx = xs[1 - kCX[q]];
+                    sy = ys[1 - kCY[q]];
+                    sz = zs[1 - kCZ[q]];
+                }
+                const uint32_t w = cx[1 - kCX[q]] + cy[1 - kCY[q]] + cz[1 - kCZ[q]];
+                v = fmaf(ld_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, nx, slot, sx, sy, sz)), p.ne_kinv,
+                         ne_shift<Q, q>(ne_unpack(p.mom_prev[w])));
+            }
+            out[(size_t)q * p.N + i] = v;
+        });
         return;
     }
     for (int q = 0; q < Q; ++q) {
@@ -279,25 +346,37 @@ hipError_t launch_init_fields(const KParams& p, bool jet, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_init_distros(const KParams& p, int Q, bool half, hipStream_t s) {
+template <int Q>
+static hipError_t init_distros_q(const KParams& p, bool jet, bool half, hipStream_t s) {
     const dim3 grid((p.N + 255) / 256);
-    if (Q == 27) {
-        if (half) hipLaunchKernelGGL((k_init_distros<27, true>), grid, dim3(256), 0, s, p);
-        else hipLaunchKernelGGL((k_init_distros<27, false>), grid, dim3(256), 0, s, p);
+    if (half && p.ne) {
+        if (jet) hipLaunchKernelGGL((k_init_distros<Q, true, true, false>), grid, dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((k_init_distros<Q, true, true, true>), grid, dim3(256), 0, s, p);
+    } else if (half) {
+        hipLaunchKernelGGL((k_init_distros<Q, true, false, false>), grid, dim3(256), 0, s, p);
     } else {
-        if (half) hipLaunchKernelGGL((k_init_distros<19, true>), grid, dim3(256), 0, s, p);
-        else hipLaunchKernelGGL((k_init_distros<19, false>), grid, dim3(256), 0, s, p);
+        hipLaunchKernelGGL((k_init_distros<Q, false, false, false>), grid, dim3(256), 0, s, p);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_init_distros(const KParams& p, int Q, bool jet, bool half, hipStream_t s) {
+    return Q == 27 ? init_distros_q<27>(p, jet, half, s) : init_distros_q<19>(p, jet, half, s);
+}
+
+template <int Q, bool JET, bool HALF, bool NE>
+static hipError_t gather_n(const KParams& p, int parity, bool local, int which, float* out, hipStream_t s) {
+    const dim3 grid((p.N + 255) / 256);
+    if (local) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0, true, NE>), grid, dim3(256), 0, s, p, which, out);
+    else if (parity) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 1, false, NE>), grid, dim3(256), 0, s, p, which, out);
+    else hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0, false, NE>), grid, dim3(256), 0, s, p, which, out);
     return hipGetLastError();
 }
 
 template <int Q, bool JET, bool HALF>
 static hipError_t gather_t(const KParams& p, int parity, bool local, int which, float* out, hipStream_t s) {
-    const dim3 grid((p.N + 255) / 256);
-    if (local) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0, true>), grid, dim3(256), 0, s, p, which, out);
-    else if (parity) hipLaunchKernelGGL((k_gather<Q, JET, HALF, 1, false>), grid, dim3(256), 0, s, p, which, out);
-    else hipLaunchKernelGGL((k_gather<Q, JET, HALF, 0, false>), grid, dim3(256), 0, s, p, which, out);
-    return hipGetLastError();
+    if (HALF && p.ne) return gather_n<Q, JET, true, true>(p, parity, local, which, out, s);
+    return gather_n<Q, JET, HALF, false>(p, parity, local, which, out, s);
 }
 
 template <int Q>

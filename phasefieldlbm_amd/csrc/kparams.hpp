@@ -29,6 +29,7 @@
 //   bcp : jet boundary planes, 2 planes x 13 values x nx*ny (rho, u, P, F at
 //         z = 1 and z = nz-2), written by the fused step for the boundary
 //         kernel
+//   mom : binary16 non-equilibrium form only, 2 x N x 4 binary16 (ne.hpp)
 //   gbc : jet boundary phase populations, 2 planes x nx*ny: the g_5 entering
 //         z = 1 and the g_6 entering z = nz-2 (src/BoundaryConditions.cuh:104,
 //         168)
@@ -65,6 +66,9 @@ static_assert(kThreads <= 1024, "workgroup too large");
 constexpr int kTXNarrow = 64;
 static_assert(kThreads % kTXNarrow == 0, "narrow tile");
 constexpr int tile_width(int nx) { return nx <= kTXNarrow ? kTXNarrow : kTX; }
+
+// Scale K of the binary16 non-equilibrium form (ne.hpp).
+constexpr float kNeK = 4096.0f;
 
 constexpr int kNumFields = 18;
 constexpr int kPlaneVals = 13;  // rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, fsx, fsy, fsz
@@ -107,6 +111,15 @@ struct KParams {
     float mlp_lo, mlp_hi;   // phi window of the closure
     float mlp_kmin, mlp_kmax;  // FD-curvature window of the closure
     float* kappa;              // closure curvature at interface cells [N]
+    // binary16 non-equilibrium form (ne != 0; f and h in binary16): a
+    // population is stored as ne_k (f~ - S_q(M)), S_q the shift of the cell
+    // that wrote it, and M = (A, Bx, By, Bz) of every cell is kept as four
+    // binary16 (ne.hpp).  mom_prev holds the M the populations now stored
+    // were written with, mom_cur receives this step's; the host swaps them.
+    int ne;
+    float ne_k, ne_kinv;
+    const uint2* mom_prev;
+    uint2* mom_cur;
 };
 
 // Launchers; all return hipError_t of the launch.
@@ -120,7 +133,7 @@ hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hi
 // rho = 1, the jet's nozzle plane (phi, u_z); the droplet profile is
 // uploaded by the host
 hipError_t launch_init_fields(const KParams& p, bool jet, hipStream_t s);
-hipError_t launch_init_distros(const KParams& p, int Q, bool half, hipStream_t s);
+hipError_t launch_init_distros(const KParams& p, int Q, bool jet, bool half, hipStream_t s);
 hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, bool local, int which,
                          float* out, hipStream_t s);
 hipError_t launch_diagnostics(const KParams& p, double* partials, int nblocks, hipStream_t s);

@@ -28,8 +28,9 @@ struct pf_ctx {
     size_t N = 0;
     size_t f_elems = 0;  // population array size (elements, row-interleaved layout)
     int Q = 19;
-    bool jet = true, half = false;
+    bool jet = true, half = false, ne = false;
     void* f = nullptr;
+    uint2* mom[2] = {nullptr, nullptr};  // non-equilibrium form: M of every cell, previous / current step
     void* h = nullptr;  // phase flux, 3 x N (FP32, or binary16 in the binary16 form)
     float* fld[pf::kNumFields] = {};
     float* phi_next = nullptr;  // second phi buffer, swapped with fld[PF_FIELD_PHI] every step
@@ -256,7 +257,14 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     const pf_config& g = *cfg;
     if (g.velocity_set != PF_D3Q19 && g.velocity_set != PF_D3Q27) return PF_ERR_CONFIG;
     if (g.flow_case != PF_JET && g.flow_case != PF_DROPLET) return PF_ERR_CONFIG;
-    if (g.storage != PF_STORE_FP32 && g.storage != PF_STORE_FP16) return PF_ERR_CONFIG;
+    if (g.storage != PF_STORE_FP32 && g.storage != PF_STORE_FP16 && g.storage != PF_STORE_FP16_NE) return PF_ERR_CONFIG;
+    if (g.storage == PF_STORE_FP16_NE) {
+        // pair collision only; the nozzle must lie inside the lateral ghost
+        // shell, where the boundary kernel's source columns need no wrap
+        if (g.kernel == PF_KERNEL_LITERAL) return PF_ERR_CONFIG;
+        if (g.flow_case == PF_JET && ((g.nx - 1) * 0.5 - g.diam / 2 < 1.0 || (g.ny - 1) * 0.5 - g.diam / 2 < 1.0))
+            return PF_ERR_CONFIG;
+    }
     if (g.closure != PF_CLOSURE_FD && g.closure != PF_CLOSURE_MLP) return PF_ERR_CONFIG;
     if (g.nx < 4 || g.ny < 4 || g.nz < 4) return PF_ERR_CONFIG;
     if (g.flow_case == PF_JET && (g.nz < 12 || g.diam < 1 || !(g.weber > 0) || !(g.reynolds_water > 0) ||
@@ -278,7 +286,8 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     c->N = N;
     c->Q = g.velocity_set;
     c->jet = g.flow_case == PF_JET;
-    c->half = g.storage == PF_STORE_FP16;
+    c->half = g.storage != PF_STORE_FP32;
+    c->ne = g.storage == PF_STORE_FP16_NE;
     // Row-interleaved population layout: row pitch S*nx plus `slot_pad`
     // elements (default none), plane pitch = row pitch * ny.  In-plane byte
     // offsets are 32-bit in the kernels.
@@ -334,6 +343,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     ok = ok && alloc((void**)&c->partials, (size_t)c->nred * 6 * sizeof(double));
     ok = ok && alloc((void**)&c->dout, 8 * sizeof(double));
     if (g.closure == PF_CLOSURE_MLP) ok = ok && alloc((void**)&c->kappa, N * sizeof(float));
+    if (c->ne) ok = ok && alloc((void**)&c->mom[0], N * sizeof(uint2)) && alloc((void**)&c->mom[1], N * sizeof(uint2));
     for (int i = 0; ok && i < PF_NUM_FIELDS - PF_NUM_BASE_FIELDS; ++i) {
         const int id = PF_NUM_BASE_FIELDS + i;
         const bool want = (id <= PF_FIELD_AVG_UZ && (g.derived & PF_DERIVED_AVERAGE)) ||
@@ -354,6 +364,11 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     k.bcp = c->bcp;
     k.gbc = c->gbc;
     k.kappa = c->kappa;
+    k.ne = c->ne ? 1 : 0;
+    k.ne_k = pf::kNeK;
+    k.ne_kinv = 1.0f / pf::kNeK;
+    k.mom_prev = c->mom[0];
+    k.mom_cur = c->mom[1];
     k.nx = g.nx;
     k.ny = g.ny;
     k.nz = g.nz;
@@ -385,6 +400,7 @@ void pf_destroy(pf_ctx* c) {
     (void)hipFree(c->dout);
     (void)hipFree(c->gather_buf);
     (void)hipFree(c->kappa);
+    for (auto* p : c->mom) (void)hipFree(p);
     for (auto* p : c->der) (void)hipFree(p);
     (void)hipFree(c->mlp_w);
     (void)hipFree(c->mlp_b);
@@ -399,7 +415,7 @@ size_t pf_device_bytes(const pf_ctx* c) { return c ? c->bytes : 0; }
 int pf_init_distros(pf_ctx* c) {
     if (!c) return PF_ERR_CONFIG;
     PF_HIP(c, hipSetDevice(c->dev));
-    PF_HIP(c, pf::launch_init_distros(c->kp, c->Q, c->half, c->stream));
+    PF_HIP(c, pf::launch_init_distros(c->kp, c->Q, c->jet, c->half, c->stream));
     const size_t plane = (size_t)c->cfg.nx * c->cfg.ny;
     PF_HIP(c, hipMemsetAsync(c->bcp, 0, 2 * pf::kPlaneVals * plane * sizeof(float), c->stream));
     PF_HIP(c, hipMemsetAsync(c->gbc, 0, 2 * plane * sizeof(float), c->stream));
@@ -472,6 +488,15 @@ static void swap_phi(pf_ctx* c) {
     std::swap(c->fld[PF_FIELD_PHI], c->phi_next);
     c->kp.fld[PF_FIELD_PHI] = c->fld[PF_FIELD_PHI];
     c->kp.phi_next = c->phi_next;
+}
+
+// Non-equilibrium form: the M this step wrote become those the next step
+// reads.
+static void swap_mom(pf_ctx* c) {
+    if (!c->ne) return;
+    std::swap(c->mom[0], c->mom[1]);
+    c->kp.mom_prev = c->mom[0];
+    c->kp.mom_cur = c->mom[1];
 }
 
 // The kernels of one step before the boundary conditions: the phase sum, the
@@ -549,6 +574,7 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
             if (rc != PF_OK) return rc;
         }
         if (c->jet) PF_HIP(c, pf::launch_jet_bc(c->kp, c->Q, c->half, c->parity ^ 1, c->stream));
+        swap_mom(c);
         if (avg) PF_HIP(c, pf::launch_average(c->kp, dp, (float)(t0 + s + 1), c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[3], c->stream));
         c->parity ^= 1;

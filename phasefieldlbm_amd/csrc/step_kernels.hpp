@@ -33,6 +33,7 @@
 
 #include "kparams.hpp"
 #include "lattice.hpp"
+#include "ne.hpp"
 #include "phase_flux.hpp"
 
 namespace pf {
@@ -422,8 +423,9 @@ __device__ __forceinline__ void pair_out(const Coll& k, float ux, float uy, floa
 #define PF_SLIM_LDS_FP16 1
 #endif
 
-template <int Q, bool JET, bool HALF, int PAR, bool MLP, bool LIT, bool SAVE, int TX>
+template <int Q, bool JET, bool HALF, int PAR, bool MLP, bool LIT, bool SAVE, int TX, bool NE = false>
 __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
+    static_assert(!NE || (HALF && !LIT), "the non-equilibrium form is binary16 with the pair collision");
     constexpr int TY = kThreads / TX;                  // rows of the TX x TY tile
     constexpr int HX = TX + 4, HY = TY + 4;            // phi tile with 2-cell halo
     constexpr int NX1 = TX + 2, NY1 = TY + 2;          // normal tile with 1-cell halo
@@ -433,6 +435,10 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     __shared__ float s_phi[kRing][HY * HX];
     __shared__ float s_n[3][kNC][NY1 * NX1];
     __shared__ float s_ind[SLIM ? 2 : 1][SLIM ? kThreads : 1];
+    // non-equilibrium form: M of planes z-1, z, z+1 over the tile and a
+    // 1-cell halo, taken at the wrapped coordinates of the cells that wrote
+    // the populations each cell reads (ne.hpp)
+    __shared__ uint2 s_m[NE ? 3 : 1][NE ? NX1 * NY1 : 1];
     constexpr int NP = (Q - 1) / 2;
 
     const int tid = threadIdx.x;
@@ -471,6 +477,37 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             const int i = tid + j * kThreads;
             const int gx = x0 - 2 + i % HX, gy = y0 - 2 + i / HX;
             if (i < HX * HY) dst[i] = (zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? v[j] : 0.0f;
+        }
+    };
+
+    // M of plane pz over the 1-halo tile (non-equilibrium form), as fetch_phi
+    // and put_phi: x and y with the ghost-shell wrap (a halo cell outside
+    // the grid belongs to a ghost column that reads nothing), z wrapped for
+    // the droplet
+    constexpr int kMIt = NE ? (NX1 * NY1 + kThreads - 1) / kThreads : 1;
+    auto fetch_m = [&](int pz, uint2 (&v)[kMIt]) {
+        if constexpr (NE) {
+            int cz = pz;
+            if constexpr (!JET) cz = (pz < 1) ? nz - 2 : (pz > nz - 2 ? 1 : pz);
+            cz = min(max(cz, 0), nz - 1);
+#pragma unroll
+            for (int j = 0; j < kMIt; ++j) {
+                const int i = min(tid + j * kThreads, NX1 * NY1 - 1);
+                const int gx = x0 - 1 + i % NX1, gy = y0 - 1 + i / NX1;
+                const int wx = gx <= 0 ? nx - 2 : (gx >= nx - 1 ? 1 : gx);
+                const int wy = gy <= 0 ? ny - 2 : (gy >= ny - 1 ? 1 : gy);
+                v[j] = p.mom_prev[(uint32_t)wx + (uint32_t)nx * ((uint32_t)wy + (uint32_t)ny * (uint32_t)cz)];
+            }
+        }
+    };
+    auto put_m = [&](int pz, const uint2 (&v)[kMIt]) {
+        if constexpr (NE) {
+            uint2* dst = s_m[pz % 3];
+#pragma unroll
+            for (int j = 0; j < kMIt; ++j) {
+                const int i = tid + j * kThreads;
+                if (i < NX1 * NY1) dst[i] = v[j];
+            }
         }
     };
 
@@ -519,8 +556,13 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         float v[4][kPhiIt];
 #pragma unroll
         for (int k = 0; k < 4; ++k) fetch_phi(zs - 2 + k, v[k]);
+        uint2 mv[2][kMIt];
+        fetch_m(zs - 1, mv[0]);
+        fetch_m(zs, mv[1]);
 #pragma unroll
         for (int k = 0; k < 4; ++k) put_phi(zs - 2 + k, v[k]);
+        put_m(zs - 1, mv[0]);
+        put_m(zs, mv[1]);
     }
     lds_barrier();
     comp_normals(zs - 1);
@@ -540,6 +582,10 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
     float phi_next[kPhiIt];
     if constexpr (kPrefetch) fetch_phi(zs + 2, phi_next);
     for (int z = zs; z < ze; ++z) {
+        // M of plane z + 1 replaces that of z - 2, which the previous
+        // iteration read before the first barrier below
+        uint2 m_next[kMIt];
+        fetch_m(z + 1, m_next);
         if constexpr (kPrefetch) {
             put_phi(z + 2, phi_next);
             if (z + 1 < ze) fetch_phi(z + 3, phi_next);
@@ -548,6 +594,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             put_phi(z + 2, phi_next);
         }
         lds_barrier();
+        put_m(z + 1, m_next);
         comp_normals(z + 1);
         lds_barrier();
 
@@ -679,6 +726,27 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                     fb[j] = ldf<HALF>(p, a, nb_f<a, 1>(nb));
                 }
             });
+            float f0r = f0;
+            if constexpr (NE) {
+                // In both parities the population of direction +c that
+                // reaches x was written by x - c and that of -c by x + c:
+                // add back their shifts (ne.hpp).
+                const uint2* sm[3] = {s_m[(z + 2) % 3], s_m[z % 3], s_m[(z + 1) % 3]};
+                const int o1 = (ty + 1) * NX1 + (tx + 1);
+                auto m_at = [&](auto qc, auto sgn) {
+                    constexpr int q = decltype(qc)::value, sg = decltype(sgn)::value;
+                    return ne_unpack(sm[sg * kCZ[q] + 1][o1 + sg * (kCY[q] * NX1 + kCX[q])]);
+                };
+                const float kinv = p.ne_kinv;
+                f0r = fmaf(f0, kinv, ne_shift<Q, 0>(ne_unpack(sm[1][o1])));
+                static_for<NP>([&](auto jc) {
+                    constexpr int j = decltype(jc)::value, a = 2 * j + 1, b = 2 * j + 2;
+                    const float4 ma = m_at(std::integral_constant<int, a>{}, std::integral_constant<int, -1>{});
+                    const float4 mb = m_at(std::integral_constant<int, a>{}, std::integral_constant<int, 1>{});
+                    fa[j] = fmaf(fa[j], kinv, ne_shift<Q, a>(ma));
+                    fb[j] = fmaf(fb[j], kinv, ne_shift<Q, b>(mb));
+                });
+            }
             // Moments from pair sums and differences: rho = 1 + sum f,
             // j = sum f c, Pi = sum f c c (src/lbm.cu:49-115).
             float sp[NP], dp[NP];
@@ -687,7 +755,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 sp[j] = fa[j] + fb[j];
                 dp[j] = fa[j] - fb[j];
             }
-            float m0 = f0;
+            float m0 = f0r;
 #pragma unroll
             for (int j = 0; j < NP; ++j) m0 += sp[j];
             rho = m0 + 1.0f;
@@ -748,12 +816,28 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 k.Bz = k.X2 + k.X4;
                 k.X7s = 6.0f * (ux * pyz + uy * pxz + uz * pxy);
             }
+            // non-equilibrium form: this cell's M, with which it stores its
+            // populations and its readers rebuild them next step
+            float4 mw = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if constexpr (NE) {
+                const NeM me = ne_mom(rho, ux, uy, uz);
+                mw = me.m;
+                p.mom_cur[c] = me.packed;
+            }
             // rest direction: f_eq + F_0/2 (its regularized part is -cs^2 tr P = 0)
-            stf<HALF>(p, 0, sf, wk<Q>(0) * k.B0);
+            {
+                float o0 = wk<Q>(0) * k.B0;
+                if constexpr (NE) o0 = p.ne_k * (o0 - ne_shift<Q, 0>(mw));
+                stf<HALF>(p, 0, sf, o0);
+            }
             static_for<NP>([&](auto jc) {
                 constexpr int j = decltype(jc)::value, a = 2 * j + 1, b = 2 * j + 2;
                 float op, om;
                 pair_out<Q, kCX[a], kCY[a], kCZ[a]>(k, ux, uy, uz, fsx, fsy, fsz, op, om);
+                if constexpr (NE) {
+                    op = p.ne_k * (op - ne_shift<Q, a>(mw));
+                    om = p.ne_k * (om - ne_shift<Q, b>(mw));
+                }
                 if constexpr (PAR == 0) {
                     stf<HALF>(p, b, sf, op);
                     stf<HALF>(p, a, sf, om);
@@ -851,7 +935,7 @@ __device__ __forceinline__ bool in_nozzle(const KParams& p, int x, int y) {
     return dx * dx + dy * dy <= p.R2;
 }
 
-template <int Q, bool HALF, int NPAR>
+template <int Q, bool HALF, int NPAR, bool NE = false>
 __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
 #pragma clang fp contract(off)
     const int x = 1 + blockIdx.x * kTX + (threadIdx.x & (kTX - 1));
@@ -861,6 +945,26 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
     const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
     const uint32_t pi = (uint32_t)x + (uint32_t)nx * (uint32_t)y;
     float* const* F = p.fld;
+    // wrapped lateral neighbours: the source columns of the populations that
+    // enter this column (ghost-shell periodicity)
+    const int xw3[3] = {(x == 1) ? nx - 2 : x - 1, x, (x == nx - 2) ? 1 : x + 1};
+    const int yw3[3] = {(y == 1) ? ny - 2 : y - 1, y, (y == ny - 2) ? 1 : y + 1};
+    // Non-equilibrium form: the populations entering z = 1 are written by the
+    // ghost cells of z = 0 and those entering z = nz-2 by the ghost cells of
+    // z = nz-1; their M (ne.hpp) goes into the ghost planes of mom_cur, and
+    // each stored value is taken relative to its source cell's shift.  The
+    // nozzle lies inside the lateral ghost shell, so its source columns need
+    // no wrap; the frozen wall has rho = 1, u = 0, so M = 0 and S = 0 there.
+    auto inflow_m = [&](int xs, int ys) {
+        NeM m{make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_uint2(0u, 0u)};
+        if (in_nozzle(p, xs, ys)) {
+            float ux, uy, uz;
+            nozzle_u(p, xs, ys, ux, uy, uz);
+            m = ne_mom(1.0f, ux, uy, uz);
+        }
+        return m;
+    };
+    if constexpr (NE) p.mom_cur[pi] = inflow_m(x, y).packed;
 
     // ---------------- inflow, target cell (x, y, 1) ----------------
     {
@@ -881,6 +985,8 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                     const float cu = cu_of<Q, q>(ux, uy, uz);
                     val = f_eq<Q, q>(1.0f, uu, cu) + p.omco_zmin * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz);
                 }
+                if constexpr (NE)
+                    val = p.ne_k * (val - ne_shift<Q, q>(inflow_m(xw3[1 - kCX[q]], yw3[1 - kCY[q]]).m));
                 int slot;
                 Loc l;
                 in_slot<q, NPAR>(nb, slot, l);
@@ -915,8 +1021,6 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         const uint32_t c = pi + zoff;
         const Nbr nb = make_nbr<false>(p, x, y, nz - 2);
         const float* phi_g = F[10];
-        const int xw3[3] = {(x == 1) ? nx - 2 : x - 1, x, (x == nx - 2) ? 1 : x + 1};
-        const int yw3[3] = {(y == 1) ? ny - 2 : y - 1, y, (y == ny - 2) ? 1 : y + 1};
         static_for<Q>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (kCZ[q] == -1) {
@@ -932,9 +1036,10 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                 const float omega_zmax =
                     (float)(1.0 / (0.5 + (double)kAs2 * (double)(visc_local * (1.0f + 100.0f))));
                 const float omco_zmax = 1.0f - omega_zmax;
-                const float val = f_eq<Q, q>(rho_b, uu, cu) +
-                                  omco_zmax * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz) +
-                                  0.5f * force_of<Q, q>(cu, ux, uy, uz, tfx, tfy, tfz);
+                float val = f_eq<Q, q>(rho_b, uu, cu) +
+                            omco_zmax * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz) +
+                            0.5f * force_of<Q, q>(cu, ux, uy, uz, tfx, tfy, tfz);
+                if constexpr (NE) val = p.ne_k * (val - ne_shift<Q, q>(ne_mom(rho_b, ux, uy, uz).m));
                 int slot;
                 Loc l;
                 in_slot<q, NPAR>(nb, slot, l);
@@ -948,6 +1053,7 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
         const uint32_t cg = c + plane;
         F[10][cg] = phi_b;
         p.phi_next[cg] = phi_b;
+        if constexpr (NE) p.mom_cur[cg] = ne_mom(b[0 * plane + pi], b[1 * plane + pi], b[2 * plane + pi], uz_b).packed;
         if (p.save) {
             F[0][cg] = b[0 * plane + pi];
             F[1][cg] = b[1 * plane + pi];
@@ -968,35 +1074,45 @@ static hipError_t phase_sum_t(const KParams& p, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int Q, bool JET, bool HALF, bool MLP, bool LIT, bool SAVE>
+template <int Q, bool JET, bool HALF, bool MLP, bool LIT, bool SAVE, bool NE>
 static hipError_t step_t(const KParams& p, int parity, hipStream_t s) {
     const uint32_t nblk = (uint32_t)p.ntx * (uint32_t)p.nty * (uint32_t)p.ntz;
     if (p.tile_x == kTXNarrow && kTXNarrow != kTX) {
         if (parity)
-            hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE, kTXNarrow>), dim3(nblk), dim3(kThreads), 0, s, p);
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE, kTXNarrow, NE>), dim3(nblk), dim3(kThreads), 0, s, p);
         else
-            hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE, kTXNarrow>), dim3(nblk), dim3(kThreads), 0, s, p);
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE, kTXNarrow, NE>), dim3(nblk), dim3(kThreads), 0, s, p);
     } else {
         if (parity)
-            hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE, kTX>), dim3(nblk), dim3(kThreads), 0, s, p);
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 1, MLP, LIT, SAVE, kTX, NE>), dim3(nblk), dim3(kThreads), 0, s, p);
         else
-            hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE, kTX>), dim3(nblk), dim3(kThreads), 0, s, p);
+            hipLaunchKernelGGL((k_step<Q, JET, HALF, 0, MLP, LIT, SAVE, kTX, NE>), dim3(nblk), dim3(kThreads), 0, s, p);
     }
     return hipGetLastError();
 }
 
 template <int Q, bool JET, bool HALF>
 static hipError_t step_v(const KParams& p, int parity, int variant, bool save, hipStream_t s) {
+    if constexpr (HALF) {
+        if (p.ne) {  // non-equilibrium form: pair collision, with or without the closure
+            if (variant == 1) return hipErrorInvalidValue;
+            if (variant == 2)
+                return save ? step_t<Q, JET, true, true, false, true, true>(p, parity, s)
+                            : step_t<Q, JET, true, true, false, false, true>(p, parity, s);
+            return save ? step_t<Q, JET, true, false, false, true, true>(p, parity, s)
+                        : step_t<Q, JET, true, false, false, false, true>(p, parity, s);
+        }
+    }
     if (variant == 1) {  // literal evaluation (FP32 and FP16 storage)
-        return save ? step_t<Q, JET, HALF, false, true, true>(p, parity, s)
-                    : step_t<Q, JET, HALF, false, true, false>(p, parity, s);
+        return save ? step_t<Q, JET, HALF, false, true, true, false>(p, parity, s)
+                    : step_t<Q, JET, HALF, false, true, false, false>(p, parity, s);
     }
     if (variant == 2) {
-        return save ? step_t<Q, JET, HALF, true, false, true>(p, parity, s)
-                    : step_t<Q, JET, HALF, true, false, false>(p, parity, s);
+        return save ? step_t<Q, JET, HALF, true, false, true, false>(p, parity, s)
+                    : step_t<Q, JET, HALF, true, false, false, false>(p, parity, s);
     }
-    return save ? step_t<Q, JET, HALF, false, false, true>(p, parity, s)
-                : step_t<Q, JET, HALF, false, false, false>(p, parity, s);
+    return save ? step_t<Q, JET, HALF, false, false, true, false>(p, parity, s)
+                : step_t<Q, JET, HALF, false, false, false, false>(p, parity, s);
 }
 
 template <int Q>
@@ -1009,10 +1125,16 @@ static hipError_t step_q(const KParams& p, bool jet, bool half, int parity, int 
 template <int Q, bool HALF>
 static hipError_t jet_bc_t(const KParams& p, int npar, hipStream_t s) {
     dim3 grid((p.nx - 2 + kTX - 1) / kTX, (p.ny - 2 + kTY - 1) / kTY, 1);
-    if (npar)
+    if (HALF && p.ne) {
+        if (npar)
+            hipLaunchKernelGGL((k_jet_bc<Q, true, 1, true>), grid, dim3(kThreads), 0, s, p);
+        else
+            hipLaunchKernelGGL((k_jet_bc<Q, true, 0, true>), grid, dim3(kThreads), 0, s, p);
+    } else if (npar) {
         hipLaunchKernelGGL((k_jet_bc<Q, HALF, 1>), grid, dim3(kThreads), 0, s, p);
-    else
+    } else {
         hipLaunchKernelGGL((k_jet_bc<Q, HALF, 0>), grid, dim3(kThreads), 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -58,6 +58,20 @@ def test_bad_config_rejected(field, value):
     assert e.value.status == -1
 
 
+def test_ne_storage_constraints():
+    """The binary16 non-equilibrium form takes the pair collision only, and a
+    jet nozzle inside the lateral ghost shell."""
+    cfg = pf.presets.round_jet(24, 24, 40, 6, pf.D3Q27, storage=pf.STORE_FP16_NE)
+    cfg.kernel = pf.KERNEL_LITERAL
+    with pytest.raises(pf.PfError) as e:
+        pf.Simulation(cfg)
+    assert e.value.status == -1
+    cfg = pf.presets.round_jet(12, 12, 40, 10, pf.D3Q27, storage=pf.STORE_FP16_NE)
+    with pytest.raises(pf.PfError) as e:
+        pf.Simulation(cfg)
+    assert e.value.status == -1
+
+
 def test_status_strings():
     lib = _abi.load()
     for code in range(0, -8, -1):

@@ -6,6 +6,7 @@ src/flowCase/Jet.cuh:48-64, at the benchmark grids):
   D3Q19 128x128x512, D = 16, FP32
   D3Q27 256x256x1024, D = 32, FP32
   D3Q27 256x256x1024, D = 32, binary16 storage
+  D3Q27 256x256x1024, D = 32, binary16 non-equilibrium storage
 u = 0.05, Re = 5000, We = 500, uniform +-5 % u_z inflow noise, 200 steps
 (the jet front is ~10 cells from the nozzle).
 
@@ -36,7 +37,7 @@ import numpy as np
 import pytest
 
 import phasefieldlbm_amd as pf
-from phasefieldlbm_amd import D3Q19, D3Q27, NOISE_NONE, STORE_FP16, STORE_FP32, presets
+from phasefieldlbm_amd import D3Q19, D3Q27, NOISE_NONE, STORE_FP16, STORE_FP16_NE, STORE_FP32, presets
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +47,8 @@ CASES = {
     "d3q19_128x128x512": lambda **kw: presets.round_jet(128, 128, 512, 16, D3Q19, **kw),
     "d3q27_256x256x1024": lambda **kw: presets.round_jet(256, 256, 1024, 32, D3Q27, **kw),
     "d3q27_256x256x1024_fp16": lambda **kw: presets.round_jet(256, 256, 1024, 32, D3Q27, storage=STORE_FP16, **kw),
+    "d3q27_256x256x1024_fp16_ne": lambda **kw: presets.round_jet(256, 256, 1024, 32, D3Q27, storage=STORE_FP16_NE,
+                                                                 **kw),
 }
 
 
@@ -67,7 +70,7 @@ def _boundary_flux(sim, cfg):
         # the stored flux: formed in FP32, rounded to binary16 in that form
         p = phi32[k][J]
         h = np.float32(0.125) * (np.float32(4) * p * uz32[k][J] + gam * p * (np.float32(1) - p) * nz32[k][J])
-        if cfg.storage == STORE_FP16:
+        if cfg.storage in (STORE_FP16, STORE_FP16_NE):
             h = h.astype(np.float16)
         return h.astype(np.float64)
 
@@ -138,16 +141,16 @@ def test_fullsize_mirror_symmetry(case):
     }
     print(f"\n[mirror] {case}: " + ", ".join(f"{k} {v / u:.2e}" for k, v in errs.items()), flush=True)
     assert np.abs(uz).max() > 0.5 * u
-    bar = 2e-3 if cfg.storage == STORE_FP16 else 1e-5
+    bar = 2e-3 if cfg.storage in (STORE_FP16, STORE_FP16_NE) else 1e-5
     for k, v in errs.items():
         assert v <= bar * u, (k, v)
     dphi = np.abs(phi - phi[:, :, ::-1]).max()
     print(f"  phi x-mirror {dphi:.2e}", flush=True)
-    assert dphi <= (5e-3 if cfg.storage == STORE_FP16 else 1e-5)
+    assert dphi <= (5e-3 if cfg.storage in (STORE_FP16, STORE_FP16_NE) else 1e-5)
 
 
 @pytest.mark.parametrize("case,chunk", [("d3q27_256x256x1024", 4), ("d3q27_256x256x1024_fp16", 8),
-                                        ("d3q19_128x128x512", 8)])
+                                        ("d3q27_256x256x1024_fp16_ne", 8), ("d3q19_128x128x512", 8)])
 def test_fullsize_zchunk_invariance(case, chunk):
     ref, alt = CASES[case](), CASES[case]()
     alt.zchunk = chunk

@@ -15,7 +15,8 @@ least as close to the FP64 oracle as the FP32 oracle is,
 rho and phi are always held to the direct bar.
 
 The binary16 form is compared with the oracle's binary16 phase-flux mode
-(precision 17: binary16 f and h, as the library stores them).  From the same
+(precision 17: binary16 f and h, as the library stores them), the binary16
+non-equilibrium form with precision 18 (f stored as the library stores it).  From the same
 state one step agrees to one binary16 ulp per population
 (test_one_step_from_gpu_state): FP32 arithmetic that rounds a product
 differently moves a stored value across a rounding boundary now and then,
@@ -28,7 +29,8 @@ import numpy as np
 import pytest
 
 import phasefieldlbm_amd as pf
-from phasefieldlbm_amd import D3Q19, D3Q27, NOISE_GAUSSIAN_XY, NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, presets
+from phasefieldlbm_amd import (D3Q19, D3Q27, NOISE_GAUSSIAN_XY, NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, STORE_FP16_NE,
+                               presets)
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -212,11 +214,13 @@ def test_reference_droplet_roundoff_envelope():
         assert gap <= 3 * env + 1e-6, k
 
 
+@pytest.mark.parametrize("storage", [STORE_FP16, STORE_FP16_NE])
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
-def test_fp16_storage_against_fp16_oracle(Q):
-    compare(presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=STORE_FP16), 100, prec=17,
+def test_fp16_storage_against_fp16_oracle(Q, storage):
+    prec = 17 if storage == STORE_FP16 else 18
+    compare(presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=storage), 100, prec=prec,
             tol=FP16_TOL)
-    compare(presets.round_jet(20, 20, 40, 6, Q, storage=STORE_FP16, noise=NOISE_UNIFORM_Z), 60, prec=17,
+    compare(presets.round_jet(20, 20, 40, 6, Q, storage=storage, noise=NOISE_UNIFORM_Z), 60, prec=prec,
             tol=FP16_TOL)
 
 
@@ -260,8 +264,15 @@ def _one_step_case(case, storage):
     return presets.round_jet(24, 24, 48, 6, D3Q27, storage=storage, noise=NOISE_UNIFORM_Z), 120
 
 
+def _lattice(Q):
+    lib = oracle.load()
+    cx = np.zeros(Q, np.int32); cy = np.zeros(Q, np.int32); cz = np.zeros(Q, np.int32); w = np.zeros(Q)
+    assert lib.pfo_lattice(Q, cx.ctypes.data, cy.ctypes.data, cz.ctypes.data, w.ctypes.data) == 0
+    return np.stack([cx, cy, cz], 1), w
+
+
 @pytest.mark.parametrize("extra", [0, 1])
-@pytest.mark.parametrize("storage", [0, STORE_FP16])
+@pytest.mark.parametrize("storage", [0, STORE_FP16, STORE_FP16_NE])
 @pytest.mark.parametrize("case", ["droplet", "jet_q19", "jet_q27", "jet_wide"])
 def test_one_step_from_gpu_state(case, storage, extra):
     """Load the GPU's state after N steps (post-streaming f, the g rebuilt
@@ -270,10 +281,15 @@ def test_one_step_from_gpu_state(case, storage, extra):
     comparisons part by round-off growth, is checked step by step, in both
     A-A parities (extra = 0, 1).  Binary16: every stored population within
     one binary16 ulp of the oracle's (precision 17), rho and phi to 1e-6;
-    FP32: populations to 1e-6 of w0 rho0, rho, u, phi to 1e-5 of their maximum."""
+    FP32: populations to 1e-6 of w0 rho0, rho, u, phi to 1e-5 of their maximum.
+    Non-equilibrium form (precision 18): every population within half a
+    binary16 spacing of each side's stored remainder K (f - S) (S the shift
+    of its writer, ne_shift_np) plus 1e-6 w0, rho and phi to 1e-6."""
+    from tests.test_oracle_physics import ne_shift_np
     cfg, n = _one_step_case(case, storage)
     n += extra
     half = storage == STORE_FP16
+    ne = storage == STORE_FP16_NE
     print(f"\n[one-step] case={case} storage={storage} after {n} steps", flush=True)
     with pf.Simulation(cfg) as sim:
         sim.init().step(n)
@@ -282,7 +298,7 @@ def test_one_step_from_gpu_state(case, storage, extra):
         sim.step(1, save=True)
         got = {k: sim.field(k) for k in pf.FIELDS}
         f1, g1 = sim.populations("f"), sim.populations("g")
-    o = oracle.Oracle(cfg, 17 if half else 32)
+    o = oracle.Oracle(cfg, 17 if half else (18 if ne else 32))
     o.set_populations("f", f0)
     o.set_populations("g", g0)
     o.set_field("phi", phi0)
@@ -296,22 +312,48 @@ def test_one_step_from_gpu_state(case, storage, extra):
         # g = phi/8 +- h16: h16 within one ulp
         hq = np.abs(og[1:][J] - 0.5 * og[0][I])
         assert np.abs(g1[J] - og[J]).max() <= 2.0 ** -10 * hq.max() + 1e-7
+    elif ne:
+        # Each side holds S + v/K with its own shift S (from its own moments,
+        # which may round to a neighbouring binary16 M) and v its binary16
+        # remainder, so the two agree to half a binary16 spacing of each
+        # side's remainder, plus the FP32 round-off of the post-collision
+        # value, formed from O(w) terms (the FP32 one-step bar, 1e-6 w0).
+        c, w = _lattice(cfg.velocity_set)
+        ofields = {k: o.field(k) for k in ("rho", "ux", "uy", "uz")}
+        zl, zh = (1, cfg.nz - 3) if cfg.flow_case == pf.JET else (0, cfg.nz - 2)
+        K = 4096.0
+
+        def spacing16(v):
+            return np.maximum(np.abs(v) * 2.0 ** -10, 2.0 ** -24)
+
+        worst = 0.0
+        for q in range(cfg.velocity_set):
+            Sg = ne_shift_np(cfg, got, q, c, w)[zl:zh].astype(np.float64)
+            So = ne_shift_np(cfg, ofields, q, c, w)[zl:zh].astype(np.float64)
+            a, b = f1[q][I][zl:zh].astype(np.float64), of[q][I][zl:zh]
+            bound = (spacing16(K * (a - Sg)) + spacing16(K * (b - So))) / (2 * K) + 1e-6 * pop_scale(cfg)
+            worst = max(worst, float((np.abs(a - b) / bound).max()))
+        print(f"  f: max {worst:.2f} of the bound", flush=True)
+        assert worst <= 1.01
+        hq = np.abs(og[1:][J] - 0.5 * og[0][I])
+        assert np.abs(g1[J] - og[J]).max() <= 2.0 ** -10 * hq.max() + 1e-7
     else:
         assert np.abs(f1[J] - of[J]).max() / pop_scale(cfg) < 1e-6
         assert np.abs(g1[J] - og[J]).max() / 0.25 < 1e-6
-    for k in ("rho", "phi") if half else CORE:
+    for k in ("rho", "phi") if (half or ne) else CORE:
         b = o.field(k)[I]
         e = float(np.abs(got[k][I].astype(np.float64) - b).max() / field_scale(cfg, k, b))
         print(f"  {k}: {e:.2e}", flush=True)
-        assert e <= (1e-6 if half else 1e-5), (k, e)
+        assert e <= (1e-6 if (half or ne) else 1e-5), (k, e)
 
 
+@pytest.mark.parametrize("storage", [STORE_FP16, STORE_FP16_NE])
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
-def test_fp16_phase_conservation(Q):
+def test_fp16_phase_conservation(Q, storage):
     """Binary16 storage rounds f and the phase flux h, but each stored h enters
     the arriving g of one neighbour with + and of the opposite one with -, so
     the total phase is conserved as in FP32 (droplet, fully periodic)."""
-    cfg = presets.static_droplet(n=32, radius=8.0, velocity_set=Q, storage=STORE_FP16)
+    cfg = presets.static_droplet(n=32, radius=8.0, velocity_set=Q, storage=storage)
     with pf.Simulation(cfg) as sim:
         sim.init()
         m0 = sim.phase_mass()
@@ -356,7 +398,7 @@ def test_step_chunking_is_invisible():
             np.testing.assert_array_equal(a.field(k), b.field(k))
 
 
-@pytest.mark.parametrize("storage", [0, STORE_FP16])
+@pytest.mark.parametrize("storage", [0, STORE_FP16, STORE_FP16_NE])
 @pytest.mark.parametrize("zchunk", [1, 2, 3, 16, 64])
 def test_zchunk_invariance(zchunk, storage):
     """The z-marching tile length changes the schedule, not the result (both
@@ -374,7 +416,7 @@ def test_zchunk_invariance(zchunk, storage):
             np.testing.assert_array_equal(a.field(k), b.field(k))
 
 
-@pytest.mark.parametrize("storage", [0, STORE_FP16])
+@pytest.mark.parametrize("storage", [0, STORE_FP16, STORE_FP16_NE])
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
 def test_rows_of_tiles(Q, storage):
     """A grid 136 wide has two or three k_step tiles per row, so workgroups
@@ -386,6 +428,8 @@ def test_rows_of_tiles(Q, storage):
         return presets.round_jet(136, 20, 40, 6, Q, storage=storage, noise=NOISE_UNIFORM_Z)
     if storage == STORE_FP16:
         compare(mk(), 24, prec=17, tol=FP16_TOL)
+    elif storage == STORE_FP16_NE:
+        compare(mk(), 24, prec=18, tol=FP16_TOL)
     else:
         compare(mk(), 24)
 
@@ -403,7 +447,10 @@ def test_rows_of_tiles(Q, storage):
             sim.init().step(24, save=True)
             return {k: sim.field(k) for k in pf.FIELDS}
 
-    for kernel, closure in ((pf.KERNEL_PAIR, False), (pf.KERNEL_LITERAL, False), (pf.KERNEL_PAIR, True)):
+    variants = ((pf.KERNEL_PAIR, False), (pf.KERNEL_LITERAL, False), (pf.KERNEL_PAIR, True))
+    if storage == STORE_FP16_NE:  # pair collision only
+        variants = ((pf.KERNEL_PAIR, False), (pf.KERNEL_PAIR, True))
+    for kernel, closure in variants:
         ref = run(128, 0, kernel, closure)
         for tile, zchunk in ((64, 0), (128, 4), (64, 3)):
             got = run(tile, zchunk, kernel, closure)
@@ -411,7 +458,7 @@ def test_rows_of_tiles(Q, storage):
                 assert np.array_equal(got[k], ref[k]), (kernel, closure, tile, zchunk, k)
 
 
-@pytest.mark.parametrize("storage", [0, STORE_FP16])
+@pytest.mark.parametrize("storage", [0, STORE_FP16, STORE_FP16_NE])
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
 @pytest.mark.parametrize("case", ["droplet", "jet"])
 def test_tile_width_invariance(case, Q, storage):
@@ -437,7 +484,10 @@ def test_tile_width_invariance(case, Q, storage):
             return {k: sim.field(k) for k in pf.FIELDS}
 
     print(f"\n[tile] case={case} Q={Q} storage={storage}", flush=True)
-    for kernel, closure in ((pf.KERNEL_PAIR, False), (pf.KERNEL_LITERAL, False), (pf.KERNEL_PAIR, True)):
+    variants = ((pf.KERNEL_PAIR, False), (pf.KERNEL_LITERAL, False), (pf.KERNEL_PAIR, True))
+    if storage == STORE_FP16_NE:  # pair collision only
+        variants = ((pf.KERNEL_PAIR, False), (pf.KERNEL_PAIR, True))
+    for kernel, closure in variants:
         a, b = run(64, kernel, closure), run(128, kernel, closure)
         for k in pf.FIELDS:
             assert np.array_equal(a[k], b[k]), (kernel, closure, k)
@@ -521,13 +571,15 @@ def _graph_case(case):
         return presets.static_droplet(n=32, radius=8.0, velocity_set=D3Q19)
     if case == "jet_fp16":
         return presets.round_jet(24, 24, 40, 6, D3Q27, storage=STORE_FP16, noise=NOISE_UNIFORM_Z)
+    if case == "jet_fp16_ne":
+        return presets.round_jet(24, 24, 40, 6, D3Q27, storage=STORE_FP16_NE, noise=NOISE_UNIFORM_Z)
     cfg = presets.round_jet(24, 24, 40, 6, D3Q27, noise=NOISE_UNIFORM_Z)
     if case == "jet_closure":
         cfg.closure = pf.CLOSURE_MLP
     return cfg
 
 
-@pytest.mark.parametrize("case", ["droplet", "jet", "jet_fp16", "jet_closure"])
+@pytest.mark.parametrize("case", ["droplet", "jet", "jet_fp16", "jet_fp16_ne", "jet_closure"])
 def test_graph_replay_matches_direct_launches(case):
     """PF_STEP_GRAPH replays the phase sum, closure and step of each parity as
     a captured graph; the inflow kernel, which takes the time index, runs

@@ -198,3 +198,88 @@ def test_binary16_flux_mode(case):
     # it stays within binary16 round-off of the reference's FP16 mode
     ref = oracle.Oracle(cfg, 16).init().step(steps)
     assert np.abs(ref.field("phi") - phi64).max() < 2e-3
+
+
+def ne_shift_np(cfg, fields, q, c, w):
+    """Shift S_q of the binary16 non-equilibrium form, per interior cell, from
+    the moments of the cell that wrote direction q, x - c_q (ghost-shell wrap
+    in x, y; in z for the droplet): M = (A, B) = (rho (1 - 1.5|u|^2) - 1,
+    3 rho u) rounded to binary16, S_q = w_q (A + b (1 + b/2)), b = c_q.B,
+    in float32 in the library's operation order."""
+    f32 = np.float32
+    nz, ny, nx = fields["rho"].shape
+    z, y, x = np.meshgrid(np.arange(1, nz - 1), np.arange(1, ny - 1), np.arange(1, nx - 1), indexing="ij")
+
+    def wrap(i, m):
+        return np.where(i < 1, m - 2, np.where(i > m - 2, 1, i))
+
+    sx, sy = wrap(x - c[q][0], nx), wrap(y - c[q][1], ny)
+    sz = z - c[q][2] if cfg.flow_case == 0 else wrap(z - c[q][2], nz)
+    rho = fields["rho"][sz, sy, sx].astype(f32)
+    u = [fields[k][sz, sy, sx].astype(f32) for k in ("ux", "uy", "uz")]
+    v = f32(1) - f32(1.5) * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2])
+    r3 = f32(3) * rho
+    h = lambda a: a.astype(np.float16).astype(f32)
+    A = h(rho * v - f32(1))
+    B = [h(r3 * u[a]) for a in range(3)]
+    if q == 0:
+        return f32(w[q]) * A
+    b = None
+    for a in range(3):
+        if c[q][a] == 0:
+            continue
+        t = B[a] if c[q][a] > 0 else -B[a]
+        b = t if b is None else b + t
+    return f32(w[q]) * (A + b * (f32(1) + f32(0.5) * b))
+
+
+@pytest.mark.parametrize("case", ["droplet", "jet"])
+def test_ne_storage_mode(case, oracle_lib):
+    """Precision 18 (the library's binary16 non-equilibrium form): every
+    population held after a step is S_q(M of its writer) + v / 2^12 with v a
+    binary16 value; taking the reader's own moments instead of the writer's
+    leaves values off that grid.  It stays within binary16 round-off of the
+    FP32 oracle and conserves the droplet's phase."""
+    if case == "droplet":
+        cfg = presets.static_droplet(n=20, radius=5.0, velocity_set=D3Q27)
+        steps = 7
+    else:
+        cfg = presets.round_jet(16, 16, 32, 6, D3Q27, noise=NOISE_UNIFORM_Z)
+        steps = 9
+    o = oracle.Oracle(cfg, 18).init()
+    m0 = o.field("phi")[I].sum()
+    o.step(steps)
+    fields = {k: o.field(k) for k in ("rho", "ux", "uy", "uz")}
+    f = o.populations("f")
+    Q = cfg.velocity_set
+    cx = np.zeros(Q, np.int32); cy = np.zeros(Q, np.int32); cz = np.zeros(Q, np.int32); w = np.zeros(Q)
+    assert oracle_lib.pfo_lattice(Q, cx.ctypes.data, cy.ctypes.data, cz.ctypes.data, w.ctypes.data) == 0
+    c = np.stack([cx, cy, cz], 1)
+    nz = cfg.nz
+    zlo, zhi = (1, nz - 3) if case == "jet" else (0, nz - 2)  # jet: planes 1 and nz-2 take boundary values
+    K = np.float32(4096.0)
+    off_own = 0.0
+    for q in range(Q):
+        S = ne_shift_np(cfg, fields, q, c, w)[zlo:zhi]
+        fq = f[q][I][zlo:zhi].astype(np.float32)
+        v = (fq - S) * K
+        on_grid = v.astype(np.float16).astype(np.float32)
+        spacing = np.maximum(np.abs(on_grid) * 2.0 ** -10, 2.0 ** -24)
+        assert np.all(np.abs(v - on_grid) <= 1e-3 * spacing + 4.0 * K * np.spacing(np.abs(S) + np.abs(fq))), q
+    # a planted slip: the shift of direction +z from the reader's own moments
+    q = 5
+    fq = f[q][I][zlo:zhi].astype(np.float32)
+    f32 = np.float32
+    rho = fields["rho"][I][zlo:zhi].astype(f32)
+    ux, uy, uz = (fields[k][I][zlo:zhi].astype(f32) for k in ("ux", "uy", "uz"))
+    vv = f32(1) - f32(1.5) * (ux * ux + uy * uy + uz * uz)
+    A = (rho * vv - f32(1)).astype(np.float16).astype(f32)
+    b = (f32(3) * rho * uz).astype(np.float16).astype(f32)
+    S_bad = f32(w[q]) * (A + b * (f32(1) + f32(0.5) * b))
+    v_bad = (fq - S_bad) * K
+    off = np.abs(v_bad - v_bad.astype(np.float16).astype(f32)) / np.maximum(np.abs(v_bad) * 2.0 ** -10, 2.0 ** -24)
+    assert np.mean(off > 1e-2) > 0.2
+    o32 = oracle.Oracle(cfg, 32).init().step(steps)
+    assert np.abs(o32.field("rho")[I] - o.field("rho")[I]).max() < 1e-5
+    if case == "droplet":
+        assert abs(o.field("phi")[I].sum() - m0) / m0 < 1e-6

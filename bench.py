@@ -3,8 +3,8 @@
 Workload (one "step" = one full timestep of the whole grid): round jet,
 D3Q27 regularized hydrodynamics + D3Q7 phase field, 256x256x1024 cells,
 nozzle D=32, u=0.05, Re=5000, We=500, FP32 populations.  Inputs are
-resident in HBM before timing starts.  Also reported: the FP16-storage form
-of the same workload, the same workload with the learned curvature closure
+resident in HBM before timing starts.  Also reported: the two binary16
+storage forms of the same workload, the same workload with the learned curvature closure
 (its cost in MLUPS, and its Laplace pressure error against the FD curvature
 for R = 6..20), the Laplace pressure error of static droplets
 (R = 8, 12, 16), the MLUPS of the D3Q19 128x128x512 jet, the 64^3
@@ -192,6 +192,18 @@ def main():
                                "kernel_bytes_per_launch": kernel_bytes_per_cell(c16.velocity_set, 2) * inner,
                                "kernel_traffic": profiled_traffic("fp16")}
         log(f"[bench] fp16 storage: {mlups16:.0f} MLUPS ({mlups16 / mlups:.2f}x)")
+        # binary16 non-equilibrium storage (populations relative to the
+        # writer's second-order equilibrium, 8 B of moments per cell more)
+        cne = presets.jet_d3q27_256x256x1024(pf.STORE_FP16_NE)
+        wallne, tmne, _ = run_form(pf, cne, args.steps, args.warmup)
+        mlupsne = cells * args.steps / wallne / 1e6
+        bne = presets.bytes_moved_per_update(cne)
+        out["fp16_ne_storage"] = {"value": round(mlupsne, 1), "unit": "MLUPS",
+                                  "speedup_vs_fp32": round(mlupsne / mlups, 3),
+                                  "moved_roofline_frac": round(mlupsne * 1e6 * bne / (HBM_PEAK_GBS * 1e9), 4),
+                                  "kernel_ms": round(tmne["step_ms"] / args.steps, 4),
+                                  "bytes_moved_per_update": bne}
+        log(f"[bench] fp16 non-equilibrium storage: {mlupsne:.0f} MLUPS ({mlupsne / mlups:.2f}x)")
         # Learned closure against the FD curvature on a developed jet (its
         # interface has grown for `closure_warmup` steps), same step count.
         from phasefieldlbm_amd import closure

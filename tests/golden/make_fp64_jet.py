@@ -7,7 +7,7 @@ D = 8, u = 0.05, Re = 5000, We = 500, uniform +-5 % u_z inflow noise with the
 default seed: the 128x128x512 / D = 16 accuracy case of the binary16 form at
 a quarter of the resolution, small enough for the FP64 oracle to run in a few
 minutes.  Saved: u_x, u_y, u_z and phi of the interior cells after 500,
-1000 and 2000 steps as float32 rounded to 16 mantissa bits (relative
+1000, 2000 and 5000 steps as float32 rounded to 16 mantissa bits (relative
 rounding <= 2^-17 = 7.6e-6 per value, which keeps the file at a few MB; the
 errors the fixture measures are 1e-5 and above), and the FP64 sums of phi
 and rho.  Test infrastructure: the accuracy reference of
@@ -29,7 +29,7 @@ import oracle  # noqa: E402
 from phasefieldlbm_amd import D3Q27, presets  # noqa: E402
 
 OUT = os.path.join(HERE, "fp64_jet_d3q27_40x40x128.npz")
-STEPS = (500, 1000, 2000)
+STEPS = (500, 1000, 2000, 5000)
 FIELDS = ("ux", "uy", "uz", "phi")
 MANTISSA_BITS = 16
 

@@ -6,8 +6,8 @@ nozzle D=32, u=0.05, Re=5000, We=500, FP32 populations.  Inputs are
 resident in HBM before timing starts.  Also reported: the two binary16
 storage forms of the same workload, the same workload with the learned curvature closure
 (its cost in MLUPS, and its Laplace pressure error against the FD curvature
-for R = 6..20), the Laplace pressure error of static droplets
-(R = 8, 12, 16), the MLUPS of the D3Q19 128x128x512 jet, the 64^3
+for R = 6..20), the Laplace pressure error of static droplets run
+to rest (R = 8, 12, 16, 12000 steps), the MLUPS of the D3Q19 128x128x512 jet, the 64^3
 droplet and the reference's own default 64x64x128 jet, and the CPU oracle
 timed on the host on a bounded sample.
 
@@ -59,7 +59,7 @@ def run_form(pf, cfg, steps, warmup, closure_params=None, graph=False):
     return wall, tm, d
 
 
-def laplace(pf, presets, R, steps=3000):
+def laplace(pf, presets, R, steps=12000):
     cfg = presets.static_droplet(n=64, radius=float(R), sigma=0.01)
     with pf.Simulation(cfg) as sim:
         sim.init().step(steps, save=True)
@@ -222,13 +222,16 @@ def main():
                                   "umax": diagm["umax"]}
         log(f"[bench] learned closure after {cw} steps: {mlupsm:.0f} MLUPS against {mlupsf:.0f} with FD "
             f"(cost {100 * (1 - mlupsm / mlupsf):.1f} %)")
-        # Laplace pressure jump of static droplets (64^3, 3000 steps, tau_g = 1,
-        # the closure's training regime): FD curvature against the closure
+        # Laplace pressure jump of static droplets run to rest (64^3, 12000
+        # steps, tau_g = 1, the closure's training regime): FD curvature
+        # against the closure
         rows = closure.laplace(closure.load(), radii=(6, 7, 8, 10, 12, 14, 16, 20))
         out["learned_closure"]["laplace"] = [
             {"R": r["R"], "fd_rel_err": round(r["fd"]["rel_err"], 5), "mlp_rel_err": round(r["mlp"]["rel_err"], 5),
+             "fd_signed_err": round(r["fd"]["signed_err"], 5), "mlp_signed_err": round(r["mlp"]["signed_err"], 5),
              "fd_spurious_umax": r["fd"]["spurious_umax"], "mlp_spurious_umax": r["mlp"]["spurious_umax"]}
             for r in rows]
+        out["learned_closure"]["laplace_steps"] = closure.REST_STEPS
         out["learned_closure"]["laplace_mean_rel_err"] = {
             "fd": round(float(np.mean([r["fd"]["rel_err"] for r in rows])), 5),
             "mlp": round(float(np.mean([r["mlp"]["rel_err"] for r in rows])), 5)}

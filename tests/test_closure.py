@@ -11,7 +11,10 @@ import pytest
 
 import oracle
 import oracle.mlp_ref as ref
+import phasefieldlbm_amd as pf
 from phasefieldlbm_amd import closure, presets
+
+I = (slice(1, -1),) * 3
 
 
 def _relaxed_sphere(R, n, steps):
@@ -146,29 +149,59 @@ def test_closure_pointwise_gpu():
 
 @pytest.mark.gpu
 def test_closure_laplace():
-    """Static droplets run to rest with the learned closure stay at rest and
-    obey Laplace's law, R = 6..20.  At R = 8, where the finite-difference
-    curvature is least accurate, the pressure jump is closer to 2 sigma / R
-    than with it (measured: 0.4 % against 4.0 %); at R = 12 and 16 both are
-    within a few per cent (FD 0.3 % and 2.1 %, closure 1.7 % and 3.1 %); over
-    all eight radii the closure's mean error is the lower one.  Spurious
-    velocities stay at the finite-difference level (measured 1.4e-4, 1.2e-4,
-    6.7e-5 against 2.2e-4, 9.0e-5, 6.1e-5): the closure averages the network
-    over each stencil and its point reflection, so the surface forces on
-    opposite sides of a symmetric droplet cancel (without it the droplet
-    drifted at ~1e-3)."""
-    rows = closure.laplace(closure.load(), radii=(6, 7, 8, 10, 12, 14, 16, 20), steps=3000)
+    """Static droplets run to rest (12000 steps; the jump of R >= 12 still
+    settles at 3000) with the learned closure obey Laplace's law more closely
+    than with the finite-difference curvature at every radius R = 6..20, and
+    at R = 8, 12 and 16 by at least a factor 2 (measured at rest:
+    0.93 / 0.26 / 0.24 % against 4.46 / 1.73 / 0.87 %).  Spurious velocities
+    stay within twice the finite-difference level (measured at R = 8, 12, 16:
+    2.8e-4, 1.1e-4, 7.1e-5 against 2.2e-4, 8.4e-5, 3.9e-5): the closure
+    averages the network over each stencil and its point reflection, so the
+    surface forces on opposite sides of a symmetric droplet cancel (without
+    it the droplet drifted at ~1e-3), but its pointwise scatter along the
+    interface is not a gradient as the smooth FD error is, and drives a
+    larger current at the larger radii."""
+    rows = closure.laplace(closure.load(), radii=(6, 7, 8, 10, 12, 14, 16, 20))
     by_r = {r["R"]: r for r in rows}
-    assert by_r[8]["mlp"]["rel_err"] < by_r[8]["fd"]["rel_err"], by_r[8]
     for r in rows:
-        assert r["mlp"]["rel_err"] < 0.06, r
-        assert r["mlp"]["spurious_umax"] < 3.0 * max(r["fd"]["spurious_umax"], 1e-4), r
-    # over R = 6..20 the closure's mean pressure-jump error is below FD's
-    # (measured 2.4 % against 3.6 %)
-    mean_fd = np.mean([r["fd"]["rel_err"] for r in rows])
-    mean_mlp = np.mean([r["mlp"]["rel_err"] for r in rows])
-    print(f"\n[laplace] mean |dP err| FD {mean_fd:.4f} closure {mean_mlp:.4f}", flush=True)
-    assert mean_mlp < 0.85 * mean_fd, (mean_mlp, mean_fd)
+        print(f"\n[laplace] R={r['R']}: FD {r['fd']['signed_err']:+.4f} closure {r['mlp']['signed_err']:+.4f}; "
+              f"|u| FD {r['fd']['spurious_umax']:.2e} closure {r['mlp']['spurious_umax']:.2e}", flush=True)
+        assert r["mlp"]["rel_err"] < r["fd"]["rel_err"], r
+        assert r["mlp"]["spurious_umax"] < 2.0 * max(r["fd"]["spurious_umax"], 1e-5), r
+    for R in (8, 12, 16):
+        assert by_r[R]["mlp"]["rel_err"] < 0.5 * by_r[R]["fd"]["rel_err"], by_r[R]
+        assert by_r[R]["mlp"]["rel_err"] < 0.01, by_r[R]
+
+
+@pytest.mark.gpu
+def test_closure_inside_jet():
+    """The learned closure inside a D3Q27 jet (128x128x256, D = 16, 300
+    steps): every field finite and phi in range, and the flow within a few
+    per cent of the finite-difference run (the closure only changes the
+    curvature at interface cells), but not identical to it."""
+    from phasefieldlbm_amd import D3Q27, presets
+    out = {}
+    for clos in (pf.CLOSURE_FD, pf.CLOSURE_MLP):
+        cfg = presets.round_jet(128, 128, 256, 16, D3Q27)
+        cfg.closure = clos
+        with pf.Simulation(cfg) as sim:
+            if clos == pf.CLOSURE_MLP:
+                sim.set_closure_params(closure.load())
+            sim.init().step(300, save=True)
+            out[clos] = {k: sim.field(k)[I].astype(np.float64) for k in ("ux", "uy", "uz", "phi", "rho")}
+            d = sim.diagnostics()
+        assert np.isfinite(d["umax"]) and d["umax"] < 0.2
+    a, b = out[pf.CLOSURE_MLP], out[pf.CLOSURE_FD]
+    for k, v in a.items():
+        assert np.isfinite(v).all(), k
+    assert a["phi"].min() > -1e-3 and a["phi"].max() < 1.01 and a["phi"].max() > 0.99
+    u_a = np.stack([a[k] for k in ("ux", "uy", "uz")])
+    u_b = np.stack([b[k] for k in ("ux", "uy", "uz")])
+    rel = float(np.linalg.norm(u_a - u_b) / np.linalg.norm(u_b))
+    dphi = float(np.abs(a["phi"] - b["phi"]).max())
+    print(f"\n[closure jet] rel L2 u difference {rel:.2e}, max |dphi| {dphi:.2e}", flush=True)
+    assert 0.0 < rel < 0.05
+    assert dphi > 0.0
 
 
 def test_raw_weights_match():

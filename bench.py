@@ -202,7 +202,9 @@ def main():
                                   "speedup_vs_fp32": round(mlupsne / mlups, 3),
                                   "moved_roofline_frac": round(mlupsne * 1e6 * bne / (HBM_PEAK_GBS * 1e9), 4),
                                   "kernel_ms": round(tmne["step_ms"] / args.steps, 4),
-                                  "bytes_moved_per_update": bne}
+                                  "bytes_moved_per_update": bne,
+                                  "kernel_bytes_per_launch": (2 * cne.velocity_set * 2 + 3 * 2 + 4 + 16) * inner,
+                                  "kernel_traffic": profiled_traffic("fp16_ne")}
         log(f"[bench] fp16 non-equilibrium storage: {mlupsne:.0f} MLUPS ({mlupsne / mlups:.2f}x)")
         # Learned closure against the FD curvature on a developed jet (its
         # interface has grown for `closure_warmup` steps), same step count.

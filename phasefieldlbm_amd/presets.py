@@ -66,6 +66,8 @@ WORKLOADS = {
     "jet_d3q27_128x128x512_fp16": (lambda: jet_d3q27_128x128x512(STORE_FP16), 5000),
     "jet_d3q27_256x256x1024_fp16_ne": (lambda: jet_d3q27_256x256x1024(STORE_FP16_NE), 20000),
     "jet_d3q27_128x128x512_fp16_ne": (lambda: jet_d3q27_128x128x512(STORE_FP16_NE), 5000),
+    # the 128x128x512 accuracy case at a quarter of the resolution (D = 8)
+    "jet_d3q27_40x40x128": (lambda: round_jet(40, 40, 128, 8, D3Q27), 5000),
 }
 
 

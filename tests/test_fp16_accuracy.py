@@ -1,127 +1,66 @@
-"""Accuracy of binary16 population storage against the FP64 oracle.
+"""Accuracy of the binary16 storage forms against the FP64 oracle.
 
-Two workloads, both D3Q27 round jets (u = 0.05, Re = 5000, We = 500,
-uniform +-5 % u_z inflow noise):
-  * 40x40x128, D = 8: the committed FP64 fixture
-    tests/golden/fp64_jet_d3q27_40x40x128.npz (written by
-    tests/golden/make_fp64_jet.py, a few minutes of CPU) at 500, 1000 and
-    2000 steps; these tests always run on the GPU;
-  * 128x128x512, D = 16, the benchmark's accuracy case: FP64 snapshots from
-    tests/fp64_snapshots.py (about 5 h of CPU for 5000 steps), run only when
-    PF_FP64_SNAPSHOTS names a directory holding them.
-For every snapshot step the library runs with FP16 and with FP32 storage,
-and the error over interior cells is
+Workload: D3Q27 round jet on 40x40x128 cells, D = 8, u = 0.05, Re = 5000,
+We = 500, uniform +-5 % u_z inflow noise: the 128x128x512 / D = 16 accuracy
+case at a quarter of the resolution.  The committed FP64 fixture
+tests/golden/fp64_jet_d3q27_40x40x128.npz (written by
+tests/golden/make_fp64_jet.py, 6 minutes of CPU) holds the FP64 oracle's
+fields at 500, 1000, 2000 and 5000 steps; the library runs the jet with
+FP32, binary16 and binary16 non-equilibrium storage and the error over
+interior cells is
 
     rel_l2(u)   = ||u - u64||_2 / ||u64||_2       (vector field)
     rel_l2(phi) = ||phi - phi64||_2 / ||phi64||_2
     rel_linf    = max |a - a64| / max |a64|
-    rel_l2(rho - 1), phase-mass drift (sum phi - sum phi64) / sum phi64
+    phase-mass drift (sum phi - sum phi64) / sum phi64
 
-FP32 storage is the floor set by round-off growth in a jet at Re = 5000.
-The bound is 1e-3 on u and phi, or 3x that floor where the floor itself is
-larger.  Up to 1000 steps binary16 storage holds it.  Later, the jet's
-shear-layer instability amplifies the larger binary16 rounding (about 40x
-the FP32 error at 1000 steps) until the flow decorrelates from the FP64
-trajectory: at 5000 steps the error is some 7e-2 while FP32 stays near
-2.5e-4.  That case is an expected failure and fails loudly if it passes.
+FP32 storage is the floor set by round-off growth in a jet at Re = 5000
+(measured rel_l2 u / phi: 6e-6 / 3e-6 at 500 steps, 3.0e-4 / 2.8e-4 at
+5000).  The plain binary16 form holds max(1e-3, 3x that floor) up to 1000
+steps (5.5e-4 / 3.1e-4); later the jet's shear-layer instability amplifies
+its rounding until the flow decorrelates from the FP64 trajectory (3.1e-2 /
+2.7e-2 at 5000 steps).  The non-equilibrium form stores only what the
+writer's second-order equilibrium leaves, and stays 10-20x closer:
+6.9e-5 / 3.0e-5 at 1000 steps, 1.6e-3 / 2.4e-3 at 5000.
+
+The 128x128x512 case itself is compared by tests/fp16_accuracy_report.py
+against FP64 snapshots from tests/fp64_snapshots.py (about 5 h of CPU and
+250 MB, so not a committed fixture).
 """
-import glob
 import json
 import os
 
 import numpy as np
 import pytest
 
-SNAP = os.environ.get("PF_FP64_SNAPSHOTS", "")
 FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fp64_jet_d3q27_40x40x128.npz")
 
 
-def _steps(d):
-    return sorted(int(os.path.basename(p)[5:-5]) for p in glob.glob(os.path.join(d, "meta_*.json")))
-
-
-def _metrics(got, ref):
-    I = (slice(1, -1),) * 3
-    u = np.stack([got[k][I] for k in ("ux", "uy", "uz")]).astype(np.float64)
-    u64 = np.stack([ref[k][I] for k in ("ux", "uy", "uz")]).astype(np.float64)
-    ph, ph64 = got["phi"][I].astype(np.float64), ref["phi"][I].astype(np.float64)
-    r, r64 = got["rho"][I].astype(np.float64) - 1.0, ref["rho"][I].astype(np.float64) - 1.0
-    return {
-        "rel_l2_u": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
-        "rel_linf_u": float(np.abs(u - u64).max() / np.abs(u64).max()),
-        "rel_l2_phi": float(np.linalg.norm(ph - ph64) / np.linalg.norm(ph64)),
-        "rel_linf_phi": float(np.abs(ph - ph64).max() / np.abs(ph64).max()),
-        "rel_l2_rho_minus_1": float(np.linalg.norm(r - r64) / max(np.linalg.norm(r64), 1e-30)),
-        "phase_mass_drift": float((ph.sum() - ph64.sum()) / ph64.sum()),
-    }
-
-
 def test_metrics_definition():
-    """The error measures on hand-made fields (no GPU, no snapshots)."""
+    """The error measures on hand-made fields (no GPU): interior cells only,
+    phi scaled by 1.001 gives 1e-3 in rel_l2, rel_linf and the mass drift,
+    and a change in the ghost shell changes nothing."""
     rng = np.random.default_rng(0)
     shape = (6, 5, 4)
-    ref = {k: rng.normal(size=shape).astype(np.float32) for k in ("ux", "uy", "uz")}
-    ref["phi"] = rng.uniform(size=shape).astype(np.float32)
-    ref["rho"] = (1 + 0.01 * rng.normal(size=shape)).astype(np.float32)
-    m0 = _metrics(ref, ref)
-    assert all(v == 0.0 for v in m0.values())
-    got = dict(ref)
-    got["phi"] = ref["phi"] * np.float32(1.001)
-    m = _metrics(got, ref)
-    assert abs(m["rel_l2_phi"] - 1e-3) < 1e-6 and abs(m["phase_mass_drift"] - 1e-3) < 1e-6
+    I = (slice(1, -1),) * 3
+    got = {k: rng.normal(size=shape).astype(np.float32) for k in ("ux", "uy", "uz")}
+    got["phi"] = rng.uniform(0.1, 1.0, size=shape).astype(np.float32)
+    fix = {f"{k}_7": v[I].copy() for k, v in got.items()}
+    fix["sum_phi_7"] = np.float64(got["phi"][I].astype(np.float64).sum())
+    m0 = _reduced_metrics(got, fix, 7)
+    assert all(abs(v) < 1e-12 for v in m0.values()), m0
+    got["phi"] = got["phi"] * np.float32(1.001)
+    got["ux"][0] += 5.0  # ghost shell
+    m = _reduced_metrics(got, fix, 7)
+    for k in ("rel_l2_phi", "rel_linf_phi", "phase_mass_drift"):
+        assert abs(m[k] - 1e-3) < 1e-6, (k, m)
     assert m["rel_l2_u"] == 0.0
-
-
-needs_snapshots = pytest.mark.skipif(not SNAP or not os.path.isdir(SNAP),
-                                     reason="PF_FP64_SNAPSHOTS not set (FP64 snapshots take ~5 h)")
-
-
-@pytest.fixture(scope="module")
-def report():
-    """Errors of FP16 and FP32 storage against every FP64 snapshot."""
-    import phasefieldlbm_amd as pf
-    from phasefieldlbm_amd import presets
-    steps = _steps(SNAP)
-    assert steps, f"no snapshots in {SNAP}"
-    rep = {}
-    for name, storage in (("fp16", pf.STORE_FP16), ("fp32", pf.STORE_FP32)):
-        cfg = presets.jet_d3q27_128x128x512(storage)
-        with pf.Simulation(cfg) as sim:
-            sim.init()
-            done = 0
-            for s in steps:
-                sim.step(s - done, save=True)
-                done = s
-                got = {k: sim.field(k) for k in ("rho", "ux", "uy", "uz", "phi")}
-                ref = {k: np.load(os.path.join(SNAP, f"{k}_{s}.npy")) for k in got}
-                rep.setdefault(s, {})[name] = _metrics(got, ref)
-    print(json.dumps({"fp16_accuracy_vs_fp64": rep}), flush=True)
-    return rep
 
 
 def _check(r, s):
     for k in ("rel_l2_u", "rel_l2_phi"):
         assert np.isfinite(r["fp16"][k])
         assert r["fp16"][k] <= max(1e-3, 3.0 * r["fp32"][k]), (s, k, r)
-
-
-@pytest.mark.gpu
-@needs_snapshots
-def test_fp16_storage_against_fp64_to_1000_steps(report):
-    early = [s for s in report if s <= 1000]
-    assert early, "no snapshot at or before step 1000"
-    for s in early:
-        _check(report[s], s)
-
-
-@pytest.mark.gpu
-@needs_snapshots
-@pytest.mark.xfail(strict=True, reason="binary16 storage decorrelates from the FP64 jet between 1000 and 3000 "
-                                       "steps (shear-layer instability); 5000-step error ~7e-2")
-def test_fp16_storage_against_fp64_at_5000_steps(report):
-    if 5000 not in report:
-        pytest.skip("no snapshot at step 5000")
-    _check(report[5000], 5000)
 
 
 # ---------------------------------------------------------------------------
@@ -155,14 +94,18 @@ def test_reduced_fixture_matches_oracle():
     growth after 500 steps (no GPU)."""
     import oracle
     fix = np.load(FIXTURE)
-    assert list(fix["steps"]) == [500, 1000, 2000]
+    assert list(fix["steps"]) == [500, 1000, 2000, 5000]
     o = oracle.Oracle(_reduced_config(), 32).init().step(500)
     got = {k: o.field(k) for k in ("ux", "uy", "uz", "phi")}
     m = _reduced_metrics(got, fix, 500)
     print(m)
     assert m["rel_l2_u"] < 1e-4 and m["rel_l2_phi"] < 1e-4
     # the jet has developed: phi = 1 fluid well inside the domain
-    assert fix["phi_2000"][40:].max() > 0.9 and np.abs(fix["uz_2000"]).max() > 0.04
+    for s in (2000, 5000):
+        assert fix[f"phi_{s}"][40:].max() > 0.9 and np.abs(fix[f"uz_{s}"]).max() > 0.04
+    # the phase mass grows by the inflow: sum phi increases from snapshot to snapshot
+    sums = [float(fix[f"sum_phi_{s}"]) for s in fix["steps"]]
+    assert all(b > a for a, b in zip(sums, sums[1:])), sums
 
 
 @pytest.fixture(scope="module")
@@ -186,23 +129,30 @@ def reduced_report():
 
 @pytest.mark.gpu
 def test_fp16_reduced_jet_against_fp64(reduced_report):
-    """Binary16 storage on the reduced jet against the committed FP64 fixture:
-    within max(1e-3, 3 x the FP32 floor) on u and phi up to 1000 steps, and
-    FP32 storage itself within 1e-3."""
-    for s in (500, 1000):
-        r = reduced_report[s]
+    """FP32 storage within 1e-3 of FP64 on u and phi at every snapshot (to
+    5000 steps), and binary16 storage within max(1e-3, 3 x that floor) up
+    to 1000 steps; every form keeps the phase mass within 1e-4 of FP64 up
+    to 2000 steps."""
+    for s, r in reduced_report.items():
         assert r["fp32"]["rel_l2_u"] < 1e-3 and r["fp32"]["rel_l2_phi"] < 1e-3, (s, r)
-        _check(r, s)
+        if s <= 1000:
+            _check(r, s)
+        if s <= 2000:
+            assert all(abs(v["phase_mass_drift"]) < 1e-4 for v in r.values()), (s, r)
 
 
 @pytest.mark.gpu
 def test_fp16_ne_reduced_jet_against_fp64(reduced_report):
     """The binary16 non-equilibrium form on the reduced jet: within 3e-4 of
-    FP64 on u and phi up to 1000 steps (measured 5e-5 / 2e-5 at 500,
-    7e-5 / 3e-5 at 1000), and at least 3 times closer to FP64 than the
-    plain binary16 form on u at every snapshot."""
+    FP64 on u and phi up to 1000 steps (measured 4.9e-5 / 2.0e-5 at 500,
+    6.9e-5 / 3.0e-5 at 1000), within 3e-3 / 5e-3 at 5000 steps (measured
+    1.6e-3 / 2.4e-3), and at least 3 times closer to FP64 than the plain
+    binary16 form on u and phi at every snapshot (measured 4.2-19x)."""
     for s in (500, 1000):
         r = reduced_report[s]["fp16_ne"]
         assert r["rel_l2_u"] < 3e-4 and r["rel_l2_phi"] < 3e-4, (s, r)
+    r = reduced_report[5000]["fp16_ne"]
+    assert r["rel_l2_u"] < 3e-3 and r["rel_l2_phi"] < 5e-3, r
     for s, r in reduced_report.items():
-        assert r["fp16_ne"]["rel_l2_u"] * 3 < r["fp16"]["rel_l2_u"], (s, r)
+        for k in ("rel_l2_u", "rel_l2_phi"):
+            assert r["fp16_ne"][k] * 3 < r["fp16"][k], (s, k, r)

@@ -283,7 +283,8 @@ struct Base {
 // Shift of the library's binary16 non-equilibrium form: M = (A, B) =
 // (rho (1 - 1.5|u|^2) - 1, 3 rho u), rounded to binary16, of the cell that
 // writes a population,
-// S_q(M) = w_q (A + b (1 + b/2)), b = c_q . B; the population is stored as
+// S_q(M) = w_q fma(b, fma(0.5, b, 1), A) = w_q (A + b (1 + b/2)), b = c_q . B
+// (float fma, one rounding each); the population is stored as
 // K (f~ - S_q(M)) in binary16 with K = 2^12 and read back as
 // stored / K + S_q(M).  Evaluated in the library's operation order.
 struct NeMom {
@@ -308,7 +309,7 @@ inline float ne_shift(const Lat<float>& L, int q, const NeMom& m) {
         b = any ? (c[a] > 0 ? b + comp[a] : b - comp[a]) : (c[a] > 0 ? comp[a] : -comp[a]);
         any = true;
     }
-    return w * (m.A + b * (1.0f + 0.5f * b));
+    return w * std::fma(b, std::fma(0.5f, b, 1.0f), m.A);
 }
 constexpr float kNeK = 4096.0f;
 

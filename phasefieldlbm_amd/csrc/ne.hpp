@@ -5,7 +5,7 @@
 // 3 w c.u (0.15 w at u = 0.05), so each store rounds away ~4e-5 w, and the
 // shear-layer instability amplifies that seed until the flow parts from the
 // FP64 one within a few thousand steps.  This form stores instead
-//     v_q = K (f~_q - S_q(M)),   S_q(M) = w_q (A + b + b^2 / 2),  b = c_q . B,
+//     v_q = K (f~_q - S_q(M)),   S_q(M) = w_q (A + b (1 + b / 2)),  b = c_q . B,
 // with M = (A, B) = (rho (1 - 1.5 |u|^2) - 1, 3 rho u) of the cell that
 // wrote the population: S_q is the second-order equilibrium of that cell
 // (f_eq - w of D3Q19.cuh:143-150 up to O(u^2 (rho - 1)) and the third-order
@@ -16,8 +16,10 @@
 // binary16 (8 B), and the reader of a population adds S_q of its writer
 // back:
 //     f~_q(x) = v_q / K + S_q(M(x - c_q)).
-// The shift needs only to be the same on both sides, not exact; the writer
-// and reader evaluate it from the same stored M with the same code.
+// The shift needs only to be the same on both sides, not exact: reader and
+// writer evaluate it from the same stored M in the same operation order
+// (ne_shift per direction; the writer's ne_sub_pair, two directions of an
+// opposite pair at a time, gives the same bits).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -71,7 +73,9 @@ __device__ __forceinline__ float ne_cdot(const float4& m) {
     return s;
 }
 
-// S_q(M) for direction q.
+// S_q(M) for direction q: w (A + b (1 + b/2)) as w fma(b, fma(0.5, b, 1), A),
+// with every rounding spelled out (contraction off), so that it has the
+// same bits wherever it is evaluated.
 template <int Q, int q>
 __device__ __forceinline__ float ne_shift(const float4& m) {
 #pragma clang fp contract(off)
@@ -80,8 +84,23 @@ __device__ __forceinline__ float ne_shift(const float4& m) {
         return w * m.x;
     } else {
         const float b = ne_cdot<kCX[q], kCY[q], kCZ[q]>(m);
-        return w * (m.x + b * (1.0f + 0.5f * b));
+        return w * fmaf(b, fmaf(0.5f, b, 1.0f), m.x);
     }
+}
+
+// The writer's side for an opposite pair: a = direction q (c) and b = its
+// opposite (-c), with op and om the pair's outputs scaled by K:
+//     op -= K S_a(M),   om -= K S_b(M).
+// b of the opposite direction is -b exactly, and K = 2^12 scales without
+// rounding, so the results are K (out - ne_shift) bit for bit, for one
+// c . B per pair.
+template <int Q, int q>
+__device__ __forceinline__ void ne_sub_pair(const float4& m, float& op, float& om) {
+#pragma clang fp contract(off)
+    constexpr float kw = kNeK * wq<Q>(q);
+    const float b = ne_cdot<kCX[q], kCY[q], kCZ[q]>(m);
+    op = op - kw * fmaf(b, fmaf(0.5f, b, 1.0f), m.x);
+    om = om - kw * fmaf(-b, fmaf(-0.5f, b, 1.0f), m.x);
 }
 
 }  // namespace pf

@@ -394,11 +394,12 @@ struct Coll {
 //   odd  = w (3 rho t [1 + 1.5 t^2 - 1.5|u|^2]_{D3Q27} + 1.5 s + 4.5 (1-omega) N3_{D3Q27})
 // with t = c.u and s = c.F.  This is f_eq + (1-omega) f_neq(P) + F_q/2 of the
 // reference (D3Q19.cuh:143-187, D3Q27.cuh:152-206, lbm.cu:206) split into
-// its even and odd parts in c, each evaluated once per pair.
-template <int Q, int cx, int cy, int cz>
+// its even and odd parts in c, each evaluated once per pair.  KS: both
+// outputs scaled by kNeK (folded into w; a power of two, so exactly).
+template <int Q, int cx, int cy, int cz, bool KS = false>
 __device__ __forceinline__ void pair_out(const Coll& k, float ux, float uy, float uz, float fx, float fy, float fz,
                                          float& out_p, float& out_m) {
-    constexpr float w = wk<Q>((cx != 0) + (cy != 0) + (cz != 0));
+    constexpr float w = (KS ? kNeK : 1.0f) * wk<Q>((cx != 0) + (cy != 0) + (cz != 0));
     constexpr int nz_ = (cx != 0) + (cy != 0) + (cz != 0);
     const float t = cdot<cx, cy, cz>(ux, uy, uz);
     const float s = cdot<cx, cy, cz>(fx, fy, fz);
@@ -859,17 +860,15 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             // rest direction: f_eq + F_0/2 (its regularized part is -cs^2 tr P = 0)
             {
                 float o0 = wk<Q>(0) * k.B0;
-                if constexpr (NE) o0 = p.ne_k * (o0 - ne_shift<Q, 0>(mw));
+                if constexpr (NE) o0 = kNeK * (o0 - ne_shift<Q, 0>(mw));
                 stf<HALF>(p, 0, sf, o0);
             }
             static_for<NP>([&](auto jc) {
                 constexpr int j = decltype(jc)::value, a = 2 * j + 1, b = 2 * j + 2;
                 float op, om;
-                pair_out<Q, kCX[a], kCY[a], kCZ[a]>(k, ux, uy, uz, fsx, fsy, fsz, op, om);
-                if constexpr (NE) {
-                    op = p.ne_k * (op - ne_shift<Q, a>(mw));
-                    om = p.ne_k * (om - ne_shift<Q, b>(mw));
-                }
+                // non-equilibrium form: outputs scaled by K, less K S(M) (ne.hpp)
+                pair_out<Q, kCX[a], kCY[a], kCZ[a], NE>(k, ux, uy, uz, fsx, fsy, fsz, op, om);
+                if constexpr (NE) ne_sub_pair<Q, a>(mw, op, om);
                 if constexpr (PAR == 0) {
                     stf<HALF>(p, b, sf, op);
                     stf<HALF>(p, a, sf, om);

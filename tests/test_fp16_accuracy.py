@@ -20,8 +20,12 @@ FP32 storage is the floor set by round-off growth in a jet at Re = 5000
 steps (5.5e-4 / 3.1e-4); later the jet's shear-layer instability amplifies
 its rounding until the flow decorrelates from the FP64 trajectory (3.1e-2 /
 2.7e-2 at 5000 steps).  The non-equilibrium form stores only what the
-writer's second-order equilibrium leaves, and stays 10-20x closer:
-6.9e-5 / 3.0e-5 at 1000 steps, 1.6e-3 / 2.4e-3 at 5000.
+writer's second-order equilibrium leaves, and stays 3-13x closer: 8.1e-5 /
+4.7e-5 at 1000 steps, 2.2e-4 / 5.5e-4 at 2000.  Past 2000 steps the
+comparison measures decorrelation more than precision: two builds of the
+non-equilibrium form that differ only in the float rounding of its shift
+measured 1.6e-3 / 2.4e-3 and 6.8e-3 / 5.8e-3 at 5000 steps (and 6.9e-5 /
+8.1e-5 on u at 1000), so the 5000-step bounds are loose.
 
 The 128x128x512 case itself is compared by tests/fp16_accuracy_report.py
 against FP64 snapshots from tests/fp64_snapshots.py (about 5 h of CPU and
@@ -144,15 +148,18 @@ def test_fp16_reduced_jet_against_fp64(reduced_report):
 @pytest.mark.gpu
 def test_fp16_ne_reduced_jet_against_fp64(reduced_report):
     """The binary16 non-equilibrium form on the reduced jet: within 3e-4 of
-    FP64 on u and phi up to 1000 steps (measured 4.9e-5 / 2.0e-5 at 500,
-    6.9e-5 / 3.0e-5 at 1000), within 3e-3 / 5e-3 at 5000 steps (measured
-    1.6e-3 / 2.4e-3), and at least 3 times closer to FP64 than the plain
-    binary16 form on u and phi at every snapshot (measured 4.2-19x)."""
+    FP64 on u and phi up to 1000 steps (measured 5.2e-5 / 2.2e-5 at 500,
+    8.1e-5 / 4.7e-5 at 1000), at least 3 times closer to FP64 than the
+    plain binary16 form on u and phi up to 2000 steps (measured 3.8-13x),
+    and at 5000 steps, where both have partly decorrelated, within 2e-2 and
+    closer than the plain form (measured 6.8e-3 / 5.8e-3 against 3.1e-2 /
+    2.7e-2)."""
     for s in (500, 1000):
         r = reduced_report[s]["fp16_ne"]
         assert r["rel_l2_u"] < 3e-4 and r["rel_l2_phi"] < 3e-4, (s, r)
-    r = reduced_report[5000]["fp16_ne"]
-    assert r["rel_l2_u"] < 3e-3 and r["rel_l2_phi"] < 5e-3, r
     for s, r in reduced_report.items():
         for k in ("rel_l2_u", "rel_l2_phi"):
-            assert r["fp16_ne"][k] * 3 < r["fp16"][k], (s, k, r)
+            if s <= 2000:
+                assert r["fp16_ne"][k] * 3 < r["fp16"][k], (s, k, r)
+            else:
+                assert r["fp16_ne"][k] < min(2e-2, r["fp16"][k]), (s, k, r)

@@ -204,8 +204,9 @@ def ne_shift_np(cfg, fields, q, c, w):
     """Shift S_q of the binary16 non-equilibrium form, per interior cell, from
     the moments of the cell that wrote direction q, x - c_q (ghost-shell wrap
     in x, y; in z for the droplet): M = (A, B) = (rho (1 - 1.5|u|^2) - 1,
-    3 rho u) rounded to binary16, S_q = w_q (A + b (1 + b/2)), b = c_q.B,
-    in float32 in the library's operation order."""
+    3 rho u) rounded to binary16, S_q = w_q fma(b, fma(0.5, b, 1), A),
+    b = c_q.B, in float32 in the library's operation order (each fma formed
+    in float64, where the float32 product is exact, and rounded once)."""
     f32 = np.float32
     nz, ny, nx = fields["rho"].shape
     z, y, x = np.meshgrid(np.arange(1, nz - 1), np.arange(1, ny - 1), np.arange(1, nx - 1), indexing="ij")
@@ -230,7 +231,14 @@ def ne_shift_np(cfg, fields, q, c, w):
             continue
         t = B[a] if c[q][a] > 0 else -B[a]
         b = t if b is None else b + t
-    return f32(w[q]) * (A + b * (f32(1) + f32(0.5) * b))
+    return f32(w[q]) * fma32(b, fma32(f32(0.5), b, f32(1)), A)
+
+
+def fma32(a, b, c):
+    """float32 fma: a*b exact in float64, one rounding of the sum (a double
+    rounding differs from the single one only at float64 ties, which the
+    callers' tolerances cover)."""
+    return (np.float64(a) * np.float64(b) + np.float64(c)).astype(np.float32)
 
 
 @pytest.mark.parametrize("case", ["droplet", "jet"])
@@ -275,7 +283,7 @@ def test_ne_storage_mode(case, oracle_lib):
     vv = f32(1) - f32(1.5) * (ux * ux + uy * uy + uz * uz)
     A = (rho * vv - f32(1)).astype(np.float16).astype(f32)
     b = (f32(3) * rho * uz).astype(np.float16).astype(f32)
-    S_bad = f32(w[q]) * (A + b * (f32(1) + f32(0.5) * b))
+    S_bad = f32(w[q]) * fma32(b, fma32(f32(0.5), b, f32(1)), A)
     v_bad = (fq - S_bad) * K
     off = np.abs(v_bad - v_bad.astype(np.float16).astype(f32)) / np.maximum(np.abs(v_bad) * 2.0 ** -10, 2.0 ** -24)
     assert np.mean(off > 1e-2) > 0.2

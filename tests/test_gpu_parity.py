@@ -416,6 +416,21 @@ def test_zchunk_invariance(zchunk, storage):
             np.testing.assert_array_equal(a.field(k), b.field(k))
 
 
+@pytest.mark.parametrize("case", ["jet", "droplet"])
+def test_phase_sum_wide_rows(case):
+    """FP32 grids wider than 128 cells take the row phase-sum kernel (4 cells
+    per thread, 16-byte loads, x-neighbours moved across lanes): 264 cells
+    give two tiles per row, the second partial, and the lanes holding x = 1
+    and x = nx-2 load their ghost-shell-wrapped neighbours.  Against the
+    oracle (rows of 136, one partial tile, in test_rows_of_tiles)."""
+    if case == "jet":
+        cfg = presets.round_jet(264, 12, 24, 6, D3Q19, noise=NOISE_UNIFORM_Z)
+    else:
+        cfg = presets.static_droplet(n=16, radius=5.0)
+        cfg.nx = 264
+    compare(cfg, 20)
+
+
 @pytest.mark.parametrize("storage", [0, STORE_FP16, STORE_FP16_NE])
 @pytest.mark.parametrize("Q", [D3Q19, D3Q27])
 def test_rows_of_tiles(Q, storage):

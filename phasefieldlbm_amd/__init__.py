@@ -87,12 +87,12 @@ class Config:
     reserved0: float = 0.0  # reserved: must be 0
     device: int = -1
     zchunk: int = 0
-    mlp_lo: float = 0.05
-    mlp_hi: float = 0.95
+    mlp_lo: float = 0.001
+    mlp_hi: float = 0.999
     kernel: int = KERNEL_PAIR
     slot_pad: int = -1
-    mlp_kmin: float = -0.6
-    mlp_kmax: float = -0.06
+    mlp_kmin: float = -2.0
+    mlp_kmax: float = -0.03
     derived: int = 0  # DERIVED_AVERAGE | DERIVED_REYNOLDS | DERIVED_VORTICITY
     tile_x: int = 0  # timestep tile width: 0 = automatic, 64 or 128
 

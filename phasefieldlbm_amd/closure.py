@@ -3,33 +3,43 @@
 Pipeline (no counterpart in the reference, whose curvature is always the
 finite-difference div(n) of src/phaseField.cuh:142-219):
 
-1. ``make_dataset``: static droplets and ellipsoids in 64^3 boxes, random
-   semi-axes, orientation and centre, phase profile
-   phi = 1/2 (1 + tanh(-2 d / W)) with d the signed distance to the surface
-   (exact, by closest-point projection, within 12 cells of it), W = 4.  Each
-   is relaxed for 500 timesteps by this library with surface tension off, so
-   the velocity stays zero and the shape stays the analytic one while the
-   phase profile settles into the scheme's own equilibrium.  Samples are the
-   normalised 3x3x3 phi stencils (``stencils``) of cells with
-   0.05 < phi < 0.95, labelled 4 x the analytic curvature in the reference's
-   convention, kappa = div(n) with n = grad(phi)/|grad phi| (= -2/R on a
-   sphere): kappa = 2 H at the point of the ellipsoid closest to the cell
-   centre, H the signed mean curvature.  That is the curvature of the sharp
-   interface, so sigma * kappa integrated across the diffuse one gives the
-   Laplace jump 2 sigma / R exactly.
+1. ``make_dataset``: static droplets and ellipsoids in periodic boxes
+   (64^3, 96^3 for the largest), random semi-axes, orientation and centre,
+   phase profile phi = 1/2 (1 + tanh(-2 d / W)) with d the signed distance to
+   the surface (exact, by closest-point projection, within 12 cells of it),
+   W = 4.  Each is relaxed for 500 timesteps by this library with surface
+   tension off, so the velocity stays zero and the shape stays the analytic
+   one while the phase profile settles into the scheme's own equilibrium.
+   Samples are the normalised 3x3x3 phi stencils (``stencils``, in their
+   canonical frame under the cube symmetries) of cells with
+   0.001 < phi < 0.999, the whole diffuse interface, labelled 4 x the
+   analytic curvature in the reference's convention, kappa = div(n) with
+   n = grad(phi)/|grad phi| (= -2/R on a sphere): kappa = 2 H at the point of
+   the ellipsoid closest to the cell centre, H the signed mean curvature.
+   That is the curvature of the sharp interface, so sigma * kappa integrated
+   across the diffuse one gives the Laplace jump 2 sigma / R exactly.
 2. ``train``: Adam on the mean squared error with the device kernels
-   (pf_mlp_train), random cube symmetries of the stencil as augmentation.
-3. ``laplace``: static droplets of radius 8, 12, 16 run to rest (12000
-   steps) with the FD curvature and with the closure; Laplace pressure error
-   and spurious velocity of both.
+   (pf_mlp_train).  The stencils are in their canonical frame under the 48
+   symmetries of the cube (``canonical_stencils``), as the closure pass
+   evaluates them, so the network only ever sees one orientation of each
+   orbit and needs no symmetry augmentation.
+3. ``laplace``: static droplets run to rest (12000 steps) with the FD
+   curvature and with the closure; Laplace pressure error and spurious
+   velocity of both.
 
-The semi-axes are drawn so the largest, a, lies in [6, 22] cells and the
-others are 1 to 2 times smaller but never below sqrt(5 a): the smallest
-radius of curvature of the ellipsoid, c^2 / a, then stays at 5 cells or more,
-above the interface width, where a curvature is still resolved by the
-diffuse profile.  An interface plus a margin of 2W must fit inside the
-62-cell interior of a 64^3 box, which rules out radii up to 32.  Half the
-shapes are spheres.
+The semi-axes are drawn so the largest, a, lies in [4, 32] cells and the
+others are 1 to 2 times smaller but never below sqrt(4 a): the smallest
+radius of curvature of the ellipsoid, c^2 / a, then stays at 4 cells or more,
+where a curvature is still resolved by the W = 4 diffuse profile.  Shapes with
+a > 22 are relaxed in 96^3 boxes, which hold the interface and its sampled
+tail.  Half the shapes are spheres.
+
+Why the whole interface: a curvature that switches between the network's
+value and the FD one inside the interface is not a function of phi along it,
+so the surface force it gives has a rotational part that pressure cannot
+balance.  With the exact curvature -2/R applied only where
+0.05 < phi < 0.95 a droplet's spurious currents exceeded FD's; applied where
+0.001 < phi < 0.999 they were 3-6 times smaller than FD's (R = 8-16).
 """
 from __future__ import annotations
 
@@ -46,7 +56,11 @@ WEIGHTS = os.path.join(HERE, "closure_weights.npz")
 # driver (pfsim --closure).
 WEIGHTS_RAW = os.path.join(HERE, "closure_weights.f32")
 BOX = 64
+BOX_LARGE = 96
 W = 4.0
+# Phase window of the training samples: the whole diffuse interface out to
+# where the force sigma kappa |grad phi| has fallen to ~0.4 % of its peak.
+SAMPLE_LO, SAMPLE_HI = 0.001, 0.999
 # The reference's phase relaxation time (src/constants.cuh:105, 134).  It sets
 # the sharpening strength and with it the relaxed interface profile, so the
 # closure is trained and evaluated at this value only.
@@ -62,17 +76,29 @@ def random_rotation(rng):
                      [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
 
 
+def box_for(amax):
+    """Periodic box edge for a shape whose largest semi-axis is amax: the
+    shape, its sampled tail (phi down to SAMPLE_LO, ~7 cells out for W = 4)
+    and the ghost shell fit in 64^3 up to amax = 22, in 96^3 beyond."""
+    return BOX if amax <= 22.0 else BOX_LARGE
+
+
 def draw_shape(rng, sphere=False):
-    amax = rng.uniform(6.0, 22.0)
+    """Semi-axes, rotation, centre and box edge of one training shape: the
+    largest semi-axis a uniform in [4, 32], the others a / U(1, 2), raised
+    where needed to sqrt(4 a) so that no radius of curvature (c^2 / a at the
+    tip) falls below 4 cells."""
+    amax = rng.uniform(4.0, 32.0)
     if sphere:
         axes = np.full(3, amax)
     else:
-        axes = np.maximum(amax / rng.uniform(1.0, 2.0, size=3), np.sqrt(5.0 * amax))
+        axes = np.maximum(amax / rng.uniform(1.0, 2.0, size=3), np.sqrt(4.0 * amax))
         axes[0] = amax
+    n = box_for(amax)
     rot = random_rotation(rng)
-    slack = (BOX - 2) / 2.0 - 1.0 - (axes.max() + 2 * W)
-    centre = (BOX - 1) / 2.0 + rng.uniform(-1, 1, size=3) * max(slack, 0.0)
-    return axes, rot, centre
+    slack = (n - 2) / 2.0 - 1.0 - (axes.max() + 2 * W)
+    centre = (n - 1) / 2.0 + rng.uniform(-1, 1, size=3) * max(slack, 0.0)
+    return axes, rot, centre, n
 
 
 def _body(points, rot, centre):
@@ -165,11 +191,13 @@ def ellipsoid_fields(axes, rot, centre, n=BOX):
     return phi, kap
 
 
-def stencils(phi, cells):
+def stencils(phi, cells, canonical=True):
     """27 normalised features of the 3x3x3 phi stencils at flat cell indices,
     computed in FP32 as the closure kernel does: the centre (feature 13) is
     2 phi_c - 1, the others (phi_f - phi_c) / (|g| + 1e-4) with g the
-    central-difference gradient of phi at the centre."""
+    central-difference gradient of phi at the centre.  canonical: each
+    stencil in its canonical frame under the cube symmetries
+    (``canonical_stencils``), as the closure pass evaluates it."""
     z, y, x = np.unravel_index(cells, phi.shape)
     phi = np.asarray(phi, dtype=np.float32)
 
@@ -186,32 +214,69 @@ def stencils(phi, cells):
     for f in range(27):
         dx, dy, dz = f % 3 - 1, (f // 3) % 3 - 1, f // 9 - 1
         out[:, f] = (np.float32(2.0) * pc - np.float32(1.0)) if f == 13 else (at(dx, dy, dz) - pc) * inv
+    if canonical:
+        out = canonical_stencils(out, at(1, 0, 0) - at(-1, 0, 0), at(0, 1, 0) - at(0, -1, 0),
+                                 at(0, 0, 1) - at(0, 0, -1))
     return out
 
 
-def relax_config():
-    return presets.static_droplet(n=BOX, radius=8.0, velocity_set=D3Q19, sigma=0.0, tau_g=TAU_G)
+def canonical_stencils(X, dx, dy, dz, order=True):
+    """Stencils X [n, 27] in their canonical frame under the 48 symmetries of
+    the cube, given the differences d = phi(+1) - phi(-1) across each centre
+    along x, y, z: each axis reflected so that d >= 0, the axes ordered by
+    |d|, largest first, the earlier axis first at a tie.  Canonical offset
+    (u0, u1, u2) is original offset (s_x u[r_x], s_y u[r_y], s_z u[r_z]) with
+    s the reflections and r the canonical positions of the original axes
+    (order=False: the reflections only, r the identity)."""
+    d = np.stack([dx, dy, dz], -1)
+    a = np.abs(d)
+    r = np.stack([(a[:, 1] > a[:, 0]).astype(int) + (a[:, 2] > a[:, 0]),
+                  (a[:, 0] >= a[:, 1]).astype(int) + (a[:, 2] > a[:, 1]),
+                  (a[:, 0] >= a[:, 2]).astype(int) + (a[:, 1] >= a[:, 2])], -1)
+    if not order:
+        r = np.broadcast_to(np.arange(3), r.shape)
+    s = np.where(d < 0, -1, 1)
+    rows = np.arange(X.shape[0])
+    out = np.empty_like(X)
+    for f in range(27):
+        u = np.array([f % 3 - 1, (f // 3) % 3 - 1, f // 9 - 1])
+        o = s * u[r]  # original offset of canonical feature f, per sample
+        out[:, f] = X[rows, (o[:, 0] + 1) + 3 * (o[:, 1] + 1) + 9 * (o[:, 2] + 1)]
+    return out
 
 
-def make_dataset(n_drops=4096, steps=500, per_drop=256, seed=7, concurrent=16, log=print, label="surface"):
-    """(X [n,27], T [n]) from n_drops relaxed droplets/ellipsoids.  Shapes
-    are drawn up front; their phase fields are built on a thread pool while
-    `concurrent` simulations, each on its own stream, relax earlier ones."""
+def relax_config(n=BOX):
+    return presets.static_droplet(n=n, radius=8.0, velocity_set=D3Q19, sigma=0.0, tau_g=TAU_G)
+
+
+def make_dataset(n_drops=8192, steps=500, per_drop=384, seed=7, concurrent=16, log=print, label="surface"):
+    """(X [n,27], T [n]) from n_drops relaxed droplets/ellipsoids, stencils in
+    canonical orientation.  Shapes are drawn up front; their phase fields are
+    built on a thread pool while `concurrent` simulations per box size, each
+    on its own stream, relax earlier ones."""
     from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
     shapes = [draw_shape(rng, sphere=rng.uniform() < 0.5) for _ in range(n_drops)]
-    cfg = relax_config()
-    sims = [Simulation(cfg) for _ in range(concurrent)]
+    shapes.sort(key=lambda sh: sh[3])  # by box size
+    sims = {}
     xs, ts = [], []
-    rho = np.ones(cfg.shape, dtype=np.float32)
-    zero = np.zeros(cfg.shape, dtype=np.float32)
     t0 = time.time()
     with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as pool:
         phis = pool.map(lambda sh: phi_field(*sh), shapes)
         done = 0
         while done < n_drops:
+            n = shapes[done][3]
+            if n not in sims:
+                for m in list(sims):
+                    for sim in sims.pop(m):
+                        sim.close()
+                sims[n] = [Simulation(relax_config(n)) for _ in range(concurrent)]
+            rho = np.ones((n, n, n), dtype=np.float32)
+            zero = np.zeros((n, n, n), dtype=np.float32)
             batch = []
-            for sim in sims[:min(concurrent, n_drops - done)]:
+            for sim in sims[n]:
+                if done + len(batch) >= n_drops or shapes[done + len(batch)][3] != n:
+                    break
                 shape = shapes[done + len(batch)]
                 sim.set_field("rho", rho)
                 for k in ("ux", "uy", "uz"):
@@ -224,16 +289,18 @@ def make_dataset(n_drops=4096, steps=500, per_drop=256, seed=7, concurrent=16, l
                 phi = sim.field("phi")
                 inner = np.zeros(phi.shape, dtype=bool)
                 inner[1:-1, 1:-1, 1:-1] = True
-                cand = np.flatnonzero(((phi > 0.05) & (phi < 0.95) & inner).ravel())
+                cand = np.flatnonzero(((phi > SAMPLE_LO) & (phi < SAMPLE_HI) & inner).ravel())
                 if cand.size > per_drop:
                     cand = rng.choice(cand, per_drop, replace=False)
+                axes, rot, centre, _ = shape
                 xs.append(stencils(phi, cand))
-                ts.append(4.0 * curvature_labels(*shape, cand, label=label))
+                ts.append(4.0 * curvature_labels(axes, rot, centre, cand, n, label=label))
             done += len(batch)
-            if done % 512 == 0 or done == n_drops:
+            if done % 512 < len(batch) or done == n_drops:
                 log(f"[closure] {done}/{n_drops} droplets relaxed, {time.time() - t0:.1f} s")
-    for sim in sims:
-        sim.close()
+    for group in sims.values():
+        for sim in group:
+            sim.close()
     return np.concatenate(xs), np.concatenate(ts)
 
 
@@ -246,7 +313,7 @@ def train(X, T, epochs=120, batch=512, lr=2e-3, seed=3, device=0, log=print):
     for e0 in range(0, epochs, 4):
         n = min(4, epochs - e0)
         lr_e = lr * (0.5 ** (e0 // period))
-        losses += list(net.train(n, batch=batch, lr=lr_e, seed=seed + e0))
+        losses += list(net.train(n, batch=batch, lr=lr_e, seed=seed + e0, augment=False))
         log(f"[closure] epoch {e0 + n}: mse {losses[-1]:.3e} (lr {lr_e:.1e})")
     p = net.params()
     net.close()
@@ -268,11 +335,10 @@ def load(path=WEIGHTS):
 
 
 def predict_curvature(params, X, device=0):
-    """Curvature the closure applies for stencils X [n, 27]: the network
-    averaged over each stencil and its point reflection (feature
-    f <-> 26 - f), as the timestep's closure pass evaluates it."""
+    """Curvature the closure applies for canonical stencils X [n, 27]
+    (``stencils``): one network evaluation, as the timestep's closure pass."""
     net = MLP(device, params)
-    y = 0.5 * (net.predict(X).astype(np.float64) + net.predict(np.ascontiguousarray(X[:, ::-1])).astype(np.float64))
+    y = net.predict(X).astype(np.float64)
     net.close()
     return 0.25 * y
 
@@ -314,10 +380,11 @@ def curvature_error(params, radii=(6, 8, 12, 16, 20), steps=500, n=64, velocity_
 REST_STEPS = 12000
 
 
-def laplace(params, radii=(8, 12, 16), steps=REST_STEPS, sigma=0.01, n=64, velocity_set=D3Q19):
+def laplace(params, radii=(8, 12, 16), steps=REST_STEPS, sigma=0.01, n=64, velocity_set=D3Q19, **window):
     """Laplace pressure error |dp - 2 sigma/R| / (2 sigma/R) (and its sign,
     signed_err) and max spurious |u| of static droplets run to rest, FD
-    curvature against the learned closure."""
+    curvature against the learned closure.  window: config overrides of the
+    closure's windows (mlp_lo, mlp_hi, mlp_kmin, mlp_kmax)."""
     out = []
     for R in radii:
         row = {"R": R}
@@ -325,6 +392,8 @@ def laplace(params, radii=(8, 12, 16), steps=REST_STEPS, sigma=0.01, n=64, veloc
             cfg = presets.static_droplet(n=n, radius=float(R), velocity_set=velocity_set, sigma=sigma,
                                          tau_g=TAU_G)
             cfg.closure = clos
+            for k, v in window.items():
+                setattr(cfg, k, v)
             with Simulation(cfg) as sim:
                 if clos == CLOSURE_MLP:
                     sim.set_closure(*split_params(params))
@@ -347,9 +416,9 @@ def main(argv=None):
     import argparse
     import json
     ap = argparse.ArgumentParser(description="generate data, train and evaluate the learned closure")
-    ap.add_argument("--drops", type=int, default=4096)
+    ap.add_argument("--drops", type=int, default=8192)
     ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--per-drop", type=int, default=256)
+    ap.add_argument("--per-drop", type=int, default=384)
     ap.add_argument("--epochs", type=int, default=120)
     ap.add_argument("--out", default=WEIGHTS)
     ap.add_argument("--eval", metavar="NPZ", help="only evaluate saved weights")

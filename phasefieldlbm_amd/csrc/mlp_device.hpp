@@ -81,6 +81,33 @@ __device__ __forceinline__ float feat_scale(float xm, float xp, float ym, float 
     return 1.0f / (sqrtf(gx * gx + gy * gy + gz * gz) + 1e-4f);
 }
 
+// Canonical frame of a stencil under the 48 symmetries of the cube.  The
+// curvature is invariant under them, and the closure evaluates the network on
+// one representative of each orbit: every axis is reflected so that the
+// difference across the centre along it, d = phi(+1) - phi(-1), is >= 0, and
+// the axes are ordered by |d|, largest first (the earlier axis first at a
+// tie).  Canonical offset (u0, u1, u2) is then original offset
+// (sx u[rx], sy u[ry], sz u[rz]).  A cube symmetry permutes and negates the
+// differences exactly, so a stencil and its images pick the same canonical
+// stencil (up to exact ties of |d|), and the forces on symmetric parts of an
+// interface are equal by construction with one network evaluation per cell.
+struct CanonFrame {
+    int sx, sy, sz;  // reflection of each original axis
+    int rx, ry, rz;  // canonical position of each original axis
+};
+__device__ __forceinline__ CanonFrame canon_frame(float dx, float dy, float dz) {
+    const float a = fabsf(dx), b = fabsf(dy), c = fabsf(dz);
+    CanonFrame fr;
+    fr.sx = dx < 0.0f ? -1 : 1;
+    fr.sy = dy < 0.0f ? -1 : 1;
+    fr.sz = dz < 0.0f ? -1 : 1;
+    fr.rx = (int)(b > a) + (int)(c > a);
+    fr.ry = (int)(a >= b) + (int)(c > b);
+    fr.rz = (int)(a >= c) + (int)(b >= c);
+    return fr;
+}
+__device__ __forceinline__ int canon_pick(int r, int u0, int u1, int u2) { return r == 0 ? u0 : (r == 1 ? u1 : u2); }
+
 // ---------------------------------------------------------------------------
 // inference: 16 cells per wavefront job
 // ---------------------------------------------------------------------------

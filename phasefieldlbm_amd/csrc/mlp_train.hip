@@ -396,31 +396,33 @@ __global__ __launch_bounds__(256, 4) void k_closure(KParams p) {
             const int o = ((cell >> 6) + 1) * kPX + (cell & 63) + 1;
             auto ph = [&](int dx, int dy, int dz) { return s_ph[(z + dz) % 3][o + dy * kPX + dx]; };
             float pc = 0.0f, inv = 0.0f;
+            CanonFrame fr{1, 1, 1, 0, 1, 2};
             if (valid) {
                 pc = ph(0, 0, 0);
-                inv = feat_scale(ph(-1, 0, 0), ph(1, 0, 0), ph(0, -1, 0), ph(0, 1, 0), ph(0, 0, -1), ph(0, 0, 1));
+                const float xm = ph(-1, 0, 0), xp = ph(1, 0, 0), ym = ph(0, -1, 0), yp = ph(0, 1, 0);
+                const float zm = ph(0, 0, -1), zp = ph(0, 0, 1);
+                inv = feat_scale(xm, xp, ym, yp, zm, zp);
+                // the stencil in its canonical frame under the cube
+                // symmetries: one network evaluation per cell
+                fr = canon_frame(xp - xm, yp - ym, zp - zm);
             }
-            // The curvature of the point-reflected stencil (feature f <-> 26 - f)
-            // is the same; the closure averages the network over the stencil
-            // and its reflection, so the forces on opposite sides of a
-            // symmetric interface cancel exactly.
-            float yv = 0.0f;
+            bf16x8 xh, xl;
 #pragma unroll
-            for (int pass = 0; pass < 2; ++pass) {
-                const int sg = pass == 0 ? 1 : -1;
-                bf16x8 xh, xl;
-#pragma unroll
-                for (int r = 0; r < 8; ++r) {
-                    const int f = 8 * g + r;
-                    float v = 0.0f;
-                    if (valid && f < kMlpIn)
-                        v = (f == kCentre) ? 2.0f * pc - 1.0f
-                                           : (ph(sg * feat_dx(f), sg * feat_dy(f), sg * feat_dz(f)) - pc) * inv;
-                    xh[r] = (__bf16)v;
-                    xl[r] = (__bf16)(v - (float)xh[r]);
+            for (int r = 0; r < 8; ++r) {
+                const int f = 8 * g + r;
+                float v = 0.0f;
+                if (valid && f < kMlpIn) {
+                    const int u0 = feat_dx(f), u1 = feat_dy(f), u2 = feat_dz(f);
+                    v = (f == kCentre) ? 2.0f * pc - 1.0f
+                                       : (ph(fr.sx * canon_pick(fr.rx, u0, u1, u2), fr.sy * canon_pick(fr.ry, u0, u1, u2),
+                                             fr.sz * canon_pick(fr.rz, u0, u1, u2)) -
+                                          pc) *
+                                             inv;
                 }
-                yv += 0.5f * mlp_eval16(W, xh, xl);
+                xh[r] = (__bf16)v;
+                xl[r] = (__bf16)(v - (float)xh[r]);
             }
+            const float yv = mlp_eval16(W, xh, xl);
             if (g == 0 && valid)
                 p.kappa[(uint32_t)cx + (uint32_t)nx * ((uint32_t)cy + (uint32_t)ny * (uint32_t)z)] = 0.25f * yv;
         }

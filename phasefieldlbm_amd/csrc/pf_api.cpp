@@ -225,11 +225,13 @@ int pf_config_default(pf_config* cfg, int32_t flow_case, int32_t velocity_set) {
     cfg->seed = 0x9E3779B9u;
     cfg->noise = PF_NOISE_GAUSSIAN_XY;
     cfg->noise_amp = 0.05f;
-    cfg->mlp_lo = 0.05f;
-    cfg->mlp_hi = 0.95f;
+    // the learned closure over the whole diffuse interface, on convex parts
+    // with radii of curvature from 1 to ~67 cells
+    cfg->mlp_lo = 0.001f;
+    cfg->mlp_hi = 0.999f;
     cfg->slot_pad = -1;
-    cfg->mlp_kmin = -0.6f;
-    cfg->mlp_kmax = -0.06f;
+    cfg->mlp_kmin = -2.0f;
+    cfg->mlp_kmax = -0.03f;
     cfg->axis_x = 2.0f;  // src/initialConditions.cu:74 halves the x offset
     cfg->axis_y = 1.0f;
     cfg->axis_z = 1.0f;

@@ -619,6 +619,10 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         // iteration read before the first barrier below
         uint2 m_next[kMIt];
         fetch_m(z + 1, m_next);
+        // The closure's neighbour mean read phi of plane z - 2 after the
+        // previous iteration's last barrier; the 4-plane ring puts plane
+        // z + 2 in that slot below.
+        if constexpr (SLIM && MLP) lds_barrier();
         if constexpr (kPrefetch) {
             put_phi(z + 2, phi_next);
             if (z + 1 < ze) fetch_phi(z + 3, phi_next);
@@ -650,10 +654,33 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
         if (!active) continue;
         const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
         if constexpr (MLP) {
-            // learned closure at interface cells whose FD curvature lies in
-            // the trained range (evaluated by k_closure for this step)
-            if (phi > p.mlp_lo && phi < p.mlp_hi && curvature >= p.mlp_kmin && curvature <= p.mlp_kmax)
-                curvature = p.kappa[c];
+            // Learned closure at interface cells whose FD curvature lies in
+            // the trained range: the mean of the network curvature (written
+            // by k_closure for this step at every cell with
+            // mlp_lo < phi < mlp_hi) over the cell and those of its 26
+            // neighbours that are interior interface cells.  The network is
+            // evaluated on each stencil in its canonical frame under the cube
+            // symmetries; the mean spreads what remains of its pointwise
+            // error along the interface, where an error that is not a function
+            // of phi drives spurious currents.
+            if (phi > p.mlp_lo && phi < p.mlp_hi && curvature >= p.mlp_kmin && curvature <= p.mlp_kmax) {
+                const int oc = (ty + 2) * HX + (tx + 2);
+                float ks = p.kappa[c], ws = 1.0f;
+                static_for<27>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    constexpr int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = q / 9 - 1;
+                    if constexpr (q != 13) {
+                        const float pn = s_phi[(z + kRing + dz) % kRing][oc + dy * HX + dx];
+                        const int xn = x + dx, yn = y + dy, zn = z + dz;
+                        if (pn > p.mlp_lo && pn < p.mlp_hi && xn >= 1 && xn <= nx - 2 && yn >= 1 && yn <= ny - 2 &&
+                            zn >= 1 && zn <= p.nz - 2) {
+                            ks += p.kappa[(uint32_t)xn + (uint32_t)nx * ((uint32_t)yn + (uint32_t)ny * (uint32_t)zn)];
+                            ws += 1.0f;
+                        }
+                    }
+                });
+                curvature = ks / ws;
+            }
         }
 
         float ind;

@@ -103,7 +103,8 @@ def test_mlp_training_fits():
 def test_closure_in_timestep(storage, grid):
     """k_closure + the closure variant of the fused step: the curvature the
     step applies at interface cells equals the reference network evaluated
-    on the same phi stencils (and their point reflections).  Both storage
+    on the same phi stencils, each in its canonical frame, averaged over the
+    cell and its interface neighbours.  Both storage
     forms (binary16 uses the step's slim LDS layout).  The 70 x 30 x 37 box
     gives k_closure's 64 x 4 x 8 workgroups a second, partial x tile, a
     partial row block and a last z-chunk of 3 planes."""
@@ -127,11 +128,22 @@ def test_closure_in_timestep(storage, grid):
     sel = inner & (phi > cfg.mlp_lo) & (phi < cfg.mlp_hi) & (ind > 1e-3)
     cells = np.flatnonzero(sel.ravel())
     assert cells.size > 100
-    # the closure averages the network over the stencil and its point
-    # reflection (feature f <-> 26 - f)
-    X = closure.stencils(phi, cells)
-    y = 0.5 * (ref.forward(p, X, bf=True)[0] + ref.forward(p, X[:, ::-1], bf=True)[0])
-    k_ref = 0.25 * y
+    # the closure evaluates the network once per interface cell, on its
+    # stencil in the canonical frame under the cube symmetries, and the step
+    # applies the mean over the cell and its interface neighbours
+    band = inner & (phi > cfg.mlp_lo) & (phi < cfg.mlp_hi)
+    bc = np.flatnonzero(band.ravel())
+    kap = np.zeros(phi.size)
+    kap[bc] = 0.25 * ref.forward(p, closure.stencils(phi, bc), bf=True)[0]
+    kap = kap.reshape(phi.shape)
+    ksum, cnt = np.zeros(phi.shape), np.zeros(phi.shape)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                sh = (-dz, -dy, -dx)  # value at x + d moved to x
+                ksum += np.roll(kap * band, sh, axis=(0, 1, 2))
+                cnt += np.roll(band.astype(float), sh, axis=(0, 1, 2))
+    k_ref = (ksum / np.maximum(cnt, 1.0)).ravel()[cells]
     n2 = (nv ** 2).sum(0).ravel()[cells]
     k_gpu = -(fs * nv).sum(0).ravel()[cells] / (cfg.sigma * ind.ravel()[cells] * n2)
     np.testing.assert_allclose(k_gpu, k_ref, rtol=0, atol=1e-4 * max(1.0, np.abs(k_ref).max()))

@@ -150,10 +150,10 @@ typedef struct pf_config {
     /* execution */
     int32_t device;        /* HIP device index; <0 reads GPU_INDEX like setDeviceFromEnv */
     int32_t zchunk;        /* z cells marched per workgroup; 0 = automatic */
-    float mlp_lo, mlp_hi;  /* MLP applies where mlp_lo < phi < mlp_hi */
+    float mlp_lo, mlp_hi;  /* MLP applies where mlp_lo < phi < mlp_hi (default 0.001, 0.999) */
     int32_t kernel;        /* pf_kernel */
     int32_t slot_pad;      /* elements of padding per population row (beyond Q*nx); <0 = automatic (0) */
-    float mlp_kmin, mlp_kmax; /* MLP applies where the FD curvature lies in [kmin, kmax] */
+    float mlp_kmin, mlp_kmax; /* MLP applies where the FD curvature lies in [kmin, kmax] (default -2, -0.03) */
     int32_t derived;       /* pf_derived bit mask; 0 = none (the reference's default build) */
     int32_t tile_x;        /* timestep tile width: 0 = automatic (64 when nx <= 64, else 128), 64 or 128 */
 } pf_config;
@@ -217,8 +217,14 @@ int pf_phase_mass(pf_ctx *ctx, double *mass);
  * 3x3x3 phi stencil to 4 x curvature: feature (dz+1)*9 + (dy+1)*3 + (dx+1)
  * is (phi(x+d) - phi(x)) / (|g| + 1e-4), g the central-difference gradient
  * of phi at x, except the centre (feature 13), which is 2*phi(x) - 1.  The
- * features enter the matrix cores as hi + lo BF16 pairs.  No counterpart in
- * the reference. */
+ * features enter the matrix cores as hi + lo BF16 pairs.  The timestep
+ * evaluates the network once per cell with mlp_lo < phi < mlp_hi, on the
+ * stencil in its canonical frame under the 48 symmetries of the cube (each
+ * axis reflected so that phi(+1) - phi(-1) >= 0 along it, the axes ordered
+ * by that difference's magnitude, largest first, the earlier axis first at
+ * a tie), and applies, where the FD curvature lies in
+ * [mlp_kmin, mlp_kmax], the mean over the cell and its interior neighbours
+ * in that phi window.  No counterpart in the reference. */
 int pf_set_closure(pf_ctx *ctx, const float *W1, const float *b1, const float *W2,
                    const float *b2, const float *W3, const float *b3);
 

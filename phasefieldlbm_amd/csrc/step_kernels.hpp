@@ -130,14 +130,21 @@ __device__ __forceinline__ void stf(const KParams& p, int slot, Loc l, float v) 
     Pop<HALF>::st(p.f, p.plane_f, p.elems_f, (uint32_t)p.nx, slot, l, v);
 }
 
-// Reciprocal: IEEE division by default; the hardware reciprocal (1 ulp)
-// when PF_FAST_RCP is defined.
+// Reciprocal and square root: IEEE division and the correctly rounded square
+// root by default; the hardware reciprocal and square root (1 ulp) with FAST.
+// The non-equilibrium binary16 step takes FAST: it is VALU-bound, and the
+// hardware forms took it from 2.88 to 2.74 ms on the D3Q27 256x256x1024 jet
+// (MI355X, alternated builds); the plain binary16 step measured no faster
+// with them (1.70 ms either way) and keeps the IEEE forms, as does FP32.
+template <bool FAST = false>
 __device__ __forceinline__ float recip(float x) {
-#ifdef PF_FAST_RCP
-    return __builtin_amdgcn_rcpf(x);
-#else
+    if constexpr (FAST) return __builtin_amdgcn_rcpf(x);
     return 1.0f / x;
-#endif
+}
+template <bool FAST>
+__device__ __forceinline__ float root(float x) {
+    if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
+    return sqrtf(x);
 }
 
 // Neighbours of one cell (ghost-shell periodicity applied): x, y*pitch of
@@ -567,8 +574,8 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 const float gx_ = kAs2 * iso_sum<Q, 0>(val);
                 const float gy_ = kAs2 * iso_sum<Q, 1>(val);
                 const float gz_ = kAs2 * iso_sum<Q, 2>(val);
-                ind = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_);
-                const float inv = recip(ind + 1e-9f);
+                ind = root<NE>(gx_ * gx_ + gy_ * gy_ + gz_ * gz_);
+                const float inv = recip<NE>(ind + 1e-9f);
                 nrx = gx_ * inv;
                 nry = gy_ * inv;
                 nrz = gz_ * inv;
@@ -706,7 +713,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
             const float visc_local = (1.0f - phi) * p.visc_water + phi * p.visc_oil;
             const float tau_zmax = 0.5f + kAs2 * (visc_local * (1.0f + 100.0f));
             const float tau_eff = tau_phi + r * (tau_zmax - tau_phi);
-            omega = recip(tau_eff);
+            omega = recip<NE>(tau_eff);
         } else {
             omega = p.omega_ref;
         }
@@ -830,7 +837,7 @@ __global__ __launch_bounds__(kThreads) void k_step(KParams p) {
                 if constexpr (cx * cz != 0) Pi[4] = cx * cz > 0 ? Pi[4] + sp[j] : Pi[4] - sp[j];
                 if constexpr (cy * cz != 0) Pi[5] = cy * cz > 0 ? Pi[5] + sp[j] : Pi[5] - sp[j];
             });
-            const float invRho = recip(rho);
+            const float invRho = recip<NE>(rho);
             ux = (jx + 0.5f * fsx) * invRho;
             uy = (jy + 0.5f * fsy) * invRho;
             uz = (jz + 0.5f * fsz) * invRho;

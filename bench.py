@@ -4,7 +4,7 @@ Workload (one "step" = one full timestep of the whole grid): round jet,
 D3Q27 regularized hydrodynamics + D3Q7 phase field, 256x256x1024 cells,
 nozzle D=32, u=0.05, Re=5000, We=500, FP32 populations.  Inputs are
 resident in HBM before timing starts.  Also reported: the two binary16
-storage forms of the same workload, the same workload with the learned curvature closure
+storage forms and the FP32 moment storage form of the same workload, the same workload with the learned curvature closure
 (its cost in MLUPS, and its Laplace pressure error against the FD curvature
 for R = 6..20), the Laplace pressure error of static droplets run
 to rest (R = 8, 12, 16, 12000 steps), the MLUPS of the D3Q19 128x128x512 jet, the 64^3
@@ -206,6 +206,24 @@ def main():
                                   "kernel_bytes_per_launch": (2 * cne.velocity_set * 2 + 3 * 2 + 4 + 16) * inner,
                                   "kernel_traffic": profiled_traffic("fp16_ne")}
         log(f"[bench] fp16 non-equilibrium storage: {mlupsne:.0f} MLUPS ({mlupsne / mlups:.2f}x)")
+        # moment storage (FP32 coefficients of each cell's collision output,
+        # 17 per D3Q27 cell, instead of 27 populations)
+        cmo = presets.jet_d3q27_256x256x1024(pf.STORE_MOM)
+        wallmo, tmmo, _ = run_form(pf, cmo, args.steps, args.warmup)
+        mlupsmo = cells * args.steps / wallmo / 1e6
+        bmo = presets.bytes_moved_per_update(cmo)
+        kmo = (2 * 17 * 4 + 3 * 4 + 4) * inner
+        mo_ms = tmmo["step_ms"] / args.steps
+        out["moment_storage"] = {"value": round(mlupsmo, 1), "unit": "MLUPS", "dtype": "f32",
+                                 "speedup_vs_fp32": round(mlupsmo / mlups, 3),
+                                 "moved_roofline_frac": round(mlupsmo * 1e6 * bmo / (HBM_PEAK_GBS * 1e9), 4),
+                                 "reference_scheme_roofline_frac": round(mlupsmo * 1e6 * b_alg / (HBM_PEAK_GBS * 1e9), 4),
+                                 "kernel_ms": round(mo_ms, 4), "bytes_moved_per_update": bmo,
+                                 "kernel_bytes_per_launch": kmo,
+                                 "kernel_frac": round(kmo / (mo_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "phase_sum_ms": round(tmmo["phase_sum_ms"] / args.steps, 4),
+                                 "kernel_traffic": profiled_traffic("mom")}
+        log(f"[bench] moment storage: {mlupsmo:.0f} MLUPS ({mlupsmo / mlups:.2f}x), kernel {mo_ms:.3f} ms")
         # Learned closure against the FD curvature on a developed jet (its
         # interface has grown for `closure_warmup` steps), same step count.
         from phasefieldlbm_amd import closure

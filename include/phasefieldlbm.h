@@ -41,8 +41,15 @@ enum pf_flow_case { PF_JET = 0, PF_DROPLET = 1 };
  * wrote it, w (A + b + b^2/2) with b = c.B, (A, B) = (rho (1 - 1.5|u|^2) - 1,
  * 3 rho u) of that cell, kept per cell in FP32 (16 B more per cell and step
  * written and read); the phase flux as in FP16.  Pair collision only
- * (PF_KERNEL_PAIR); no counterpart in the reference. */
-enum pf_storage { PF_STORE_FP32 = 0, PF_STORE_FP16 = 1, PF_STORE_FP16_NE = 2 };
+ * (PF_KERNEL_PAIR); no counterpart in the reference.  MOM stores, in FP32,
+ * the coefficients of the polynomial in c that each cell's regularized
+ * collision output is (10 per D3Q19 cell, 17 per D3Q27 cell, instead of Q
+ * populations), and rebuilds every arriving population from the
+ * coefficients of the cell that sent it: the reference's scheme with FP32
+ * storage and arithmetic, 136 B of f traffic per D3Q27 update instead of
+ * 216 B.  Pair collision with the FD closure only (PF_KERNEL_PAIR,
+ * PF_CLOSURE_FD); no counterpart in the reference. */
+enum pf_storage { PF_STORE_FP32 = 0, PF_STORE_FP16 = 1, PF_STORE_FP16_NE = 2, PF_STORE_MOM = 3 };
 
 /* Inflow perturbation.  GAUSSIAN_XY is the reference's Box-Muller noise with
  * sigma = 0.08*u_inf on u_x, u_y (src/BoundaryConditions.cuh:60-69).

@@ -19,10 +19,10 @@ import numpy as np
 from . import _abi
 from ._abi import (CLOSURE_FD, CLOSURE_MLP, D3Q19, D3Q27, DERIVED_AVERAGE, DERIVED_FIELDS, DERIVED_REYNOLDS,
                    DERIVED_VORTICITY, DROPLET, FIELDS, JET, KERNEL_LITERAL, KERNEL_PAIR, NOISE_GAUSSIAN_XY,
-                   NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, STORE_FP16_NE, STORE_FP32)
+                   NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, STORE_FP16_NE, STORE_FP32, STORE_MOM)
 
 __all__ = ["Config", "Simulation", "PfError", "presets", "FIELDS", "D3Q19", "D3Q27", "JET", "DROPLET",
-           "STORE_FP32", "STORE_FP16", "STORE_FP16_NE", "NOISE_GAUSSIAN_XY", "NOISE_UNIFORM_Z", "NOISE_NONE",
+           "STORE_FP32", "STORE_FP16", "STORE_FP16_NE", "STORE_MOM", "NOISE_GAUSSIAN_XY", "NOISE_UNIFORM_Z", "NOISE_NONE",
            "CLOSURE_FD", "CLOSURE_MLP", "KERNEL_PAIR", "KERNEL_LITERAL", "library_path", "copy_bandwidth",
            "stream_bandwidth", "MLP",
            "split_params", "mfma_check", "vtk", "DERIVED_FIELDS", "DERIVED_AVERAGE", "DERIVED_REYNOLDS",

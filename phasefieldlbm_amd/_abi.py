@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libphasefieldlbm_amd.so")
 # enums (include/phasefieldlbm.h)
 D3Q19, D3Q27 = 19, 27
 JET, DROPLET = 0, 1
-STORE_FP32, STORE_FP16, STORE_FP16_NE = 0, 1, 2
+STORE_FP32, STORE_FP16, STORE_FP16_NE, STORE_MOM = 0, 1, 2, 3
 NOISE_GAUSSIAN_XY, NOISE_UNIFORM_Z, NOISE_NONE = 0, 1, 2
 CLOSURE_FD, CLOSURE_MLP = 0, 1
 KERNEL_PAIR, KERNEL_LITERAL = 0, 1

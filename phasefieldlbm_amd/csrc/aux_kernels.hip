@@ -13,6 +13,7 @@
 
 #include "kparams.hpp"
 #include "lattice.hpp"
+#include "mom.hpp"
 #include "ne.hpp"
 #include "phase_flux.hpp"
 
@@ -111,6 +112,55 @@ __global__ __launch_bounds__(256) void k_init_distros(KParams p) {
     static_for<Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         st_pop<HALF>(p.f, pop_elem(p.plane_f, p.pitch_f, p.nx, q, x, y, z), f_eq<Q, q>(rho, uu, cu_of<Q, q>(ux, uy, uz)));
+    });
+}
+
+// Moment form (mom.hpp): every cell's coefficients of its own equilibrium
+// f~eq(rho, u) (no stress, no force), in both coefficient buffers; the
+// first step reads each cell's own (setDistros, src/initialConditions.cu:101-138).
+template <int Q>
+__global__ __launch_bounds__(256) void k_init_mom(KParams p) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.N) return;
+    const float rho = p.fld[0][i], ux = p.fld[1][i], uy = p.fld[2][i], uz = p.fld[3][i];
+    float cf[mom_nc<Q>()];
+    mom_coeffs<Q>(rho - 1.0f, rho, ux, uy, uz, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, cf);
+    for (int k = 0; k < mom_nc<Q>(); ++k) {
+        p.rec_cur[(size_t)k * p.N + i] = cf[k];
+        const_cast<float*>(p.rec_prev)[(size_t)k * p.N + i] = cf[k];
+    }
+}
+
+// Moment form: the populations the next step reads, rebuilt from the
+// coefficients of the cells that sent them (wrapped x - c_q), or from the
+// jet's boundary populations at z = 1 (c_z = +1) and z = nz-2 (c_z = -1), by
+// the same operations as the step (mom_pop); LOCAL: each cell's own.
+template <int Q, bool JET, bool LOCAL>
+__global__ __launch_bounds__(256) void k_gather_mom(KParams p, float* out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.N) return;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int x = (int)(i % (uint32_t)nx);
+    const int y = (int)((i / (uint32_t)nx) % (uint32_t)ny);
+    const int z = (int)(i / ((uint32_t)nx * (uint32_t)ny));
+    const bool interior = x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && z >= 1 && z <= nz - 2;
+    uint32_t cx[3] = {0, 0, 0}, cy[3] = {0, 0, 0}, cz[3] = {0, 0, 0};
+    if (interior) cell_nbr<!JET>(p, x, y, z, cx, cy, cz);
+    const uint32_t plane = (uint32_t)nx * (uint32_t)ny, pi = (uint32_t)x + (uint32_t)nx * (uint32_t)y;
+    static_for<Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        float v = 0.0f;
+        if (interior) {
+            const uint32_t w = LOCAL ? i : cx[1 - kCX[q]] + cy[1 - kCY[q]] + cz[1 - kCZ[q]];
+            if (JET && !LOCAL && kCZ[q] == 1 && z == 1) {
+                v = p.bcf[(uint32_t)((kCY[q] + 1) * 3 + kCX[q] + 1) * plane + pi];
+            } else if (JET && !LOCAL && kCZ[q] == -1 && z == nz - 2) {
+                v = p.bcf[(uint32_t)(9 + (kCY[q] + 1) * 3 + kCX[q] + 1) * plane + pi];
+            } else {
+                v = mom_pop<Q, q>([&](int k) { return p.rec_prev[(size_t)k * p.N + w]; });
+            }
+        }
+        out[(size_t)q * p.N + i] = v;
     });
 }
 
@@ -361,7 +411,21 @@ static hipError_t init_distros_q(const KParams& p, bool jet, bool half, hipStrea
 }
 
 hipError_t launch_init_distros(const KParams& p, int Q, bool jet, bool half, hipStream_t s) {
+    if (p.mom) {
+        const dim3 grid((p.N + 255) / 256);
+        if (Q == 27) hipLaunchKernelGGL((k_init_mom<27>), grid, dim3(256), 0, s, p);
+        else hipLaunchKernelGGL((k_init_mom<19>), grid, dim3(256), 0, s, p);
+        return hipGetLastError();
+    }
     return Q == 27 ? init_distros_q<27>(p, jet, half, s) : init_distros_q<19>(p, jet, half, s);
+}
+
+template <int Q, bool JET>
+static hipError_t gather_mom(const KParams& p, bool local, float* out, hipStream_t s) {
+    const dim3 grid((p.N + 255) / 256);
+    if (local) hipLaunchKernelGGL((k_gather_mom<Q, JET, true>), grid, dim3(256), 0, s, p, out);
+    else hipLaunchKernelGGL((k_gather_mom<Q, JET, false>), grid, dim3(256), 0, s, p, out);
+    return hipGetLastError();
 }
 
 template <int Q, bool JET, bool HALF, bool NE>
@@ -391,6 +455,10 @@ static hipError_t gather_q(const KParams& p, bool jet, bool half, int parity, bo
 
 hipError_t launch_gather(const KParams& p, int Q, bool jet, bool half, int parity, bool local, int which,
                          float* out, hipStream_t s) {
+    if (p.mom && which == 0) {
+        if (Q == 27) return jet ? gather_mom<27, true>(p, local, out, s) : gather_mom<27, false>(p, local, out, s);
+        return jet ? gather_mom<19, true>(p, local, out, s) : gather_mom<19, false>(p, local, out, s);
+    }
     return Q == 27 ? gather_q<27>(p, jet, half, parity, local, which, out, s)
                    : gather_q<19>(p, jet, half, parity, local, which, out, s);
 }

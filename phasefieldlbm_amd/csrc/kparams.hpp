@@ -30,6 +30,9 @@
 //         z = 1 and z = nz-2), written by the fused step for the boundary
 //         kernel
 //   mom : binary16 non-equilibrium form only, 2 x N x 4 binary16 (ne.hpp)
+//   rec : moment form only, 2 x NC x N FP32 (NC = 10 / 17 for D3Q19 / D3Q27),
+//         the coefficients of each cell's collision output (mom.hpp), in
+//         place of f
 //   gbc : jet boundary phase populations, 2 planes x nx*ny: the g_5 entering
 //         z = 1 and the g_6 entering z = nz-2 (src/BoundaryConditions.cuh:104,
 //         168)
@@ -66,6 +69,8 @@ static_assert(kThreads <= 1024, "workgroup too large");
 constexpr int kTXNarrow = 64;
 static_assert(kThreads % kTXNarrow == 0, "narrow tile");
 constexpr int tile_width(int nx) { return nx <= kTXNarrow ? kTXNarrow : kTX; }
+// The moment form's step tiles are tile_x x kMomRows cells (step_mom.hpp).
+constexpr int kMomRows = 4;
 
 // Scale K of the binary16 non-equilibrium form (ne.hpp).
 constexpr float kNeK = 4096.0f;
@@ -120,6 +125,17 @@ struct KParams {
     float ne_k, ne_kinv;
     const uint2* mom_prev;
     uint2* mom_cur;
+    // moment form (mom != 0, mom.hpp): the FP32 coefficients of every
+    // cell's collision output, slot-major [k][N]; rec_prev holds those the
+    // arriving populations are rebuilt from, rec_cur receives this step's
+    // (the host swaps them).  bcf: the jet's boundary populations for the
+    // next step, 18 x nx*ny: the 9 c_z = +1 directions entering z = 1, then
+    // the 9 c_z = -1 entering z = nz-2, slot (c_y + 1) * 3 + (c_x + 1), by
+    // target column.
+    int mom;
+    const float* rec_prev;
+    float* rec_cur;
+    float* bcf;
 };
 
 // Launchers; all return hipError_t of the launch.
@@ -130,6 +146,10 @@ hipError_t launch_phase_sum(const KParams& p, bool jet, bool local, bool half, h
 // 2 pair-symmetric with the learned curvature closure
 hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity, int variant, hipStream_t s);
 hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hipStream_t s);
+// moment form: the step (local: the first step after initialisation, which
+// reads every cell's own coefficients) and its boundary populations
+hipError_t launch_step_mom(const KParams& p, int Q, bool jet, bool local, hipStream_t s);
+hipError_t launch_jet_bc_mom(const KParams& p, int Q, hipStream_t s);
 // rho = 1, the jet's nozzle plane (phi, u_z); the droplet profile is
 // uploaded by the host
 hipError_t launch_init_fields(const KParams& p, bool jet, hipStream_t s);

@@ -28,8 +28,10 @@ struct pf_ctx {
     size_t N = 0;
     size_t f_elems = 0;  // population array size (elements, row-interleaved layout)
     int Q = 19;
-    bool jet = true, half = false, ne = false;
-    void* f = nullptr;
+    bool jet = true, half = false, ne = false, moments = false;
+    void* f = nullptr;  // populations (not allocated in the moment form)
+    float* rec[2] = {nullptr, nullptr};  // moment form: coefficients, previous / current step
+    float* bcf = nullptr;                // moment form: jet boundary populations
     uint2* mom[2] = {nullptr, nullptr};  // non-equilibrium form: M of every cell, previous / current step
     void* h = nullptr;  // phase flux, 3 x N (FP32, or binary16 in the binary16 form)
     float* fld[pf::kNumFields] = {};
@@ -170,14 +172,21 @@ void fill_constants(pf_ctx* c) {
 // 64^3 droplet 14.7k MLUPS at z-chunk 1 (496 workgroups), 13.8k at 2 (248),
 // 10.9k at 4 (128); the reference's 64x64x128 D3Q19 jet 14.2k at 4 (256),
 // 14.0k at 2 (504), 12.9k at 1.
-int tile_x(const pf_config& g) { return g.tile_x > 0 ? g.tile_x : pf::tile_width(g.nx); }
+// The moment form takes 64 x 4 tiles unless told otherwise (step_mom.hpp).
+int tile_x(const pf_config& g) {
+    if (g.tile_x > 0) return g.tile_x;
+    return g.storage == PF_STORE_MOM ? 64 : pf::tile_width(g.nx);
+}
+int tile_rows(const pf_config& g, int tw) { return g.storage == PF_STORE_MOM ? pf::kMomRows : pf::kThreads / tw; }
 
 int choose_zchunk(const pf_config& g) {
     if (g.zchunk > 0) return g.zchunk;
-    const int tw = tile_x(g), th = pf::kThreads / tw;
+    const int tw = tile_x(g), th = tile_rows(g, tw);
     const long cols = (long)((g.nx + tw - 1) / tw) * ((g.ny + th - 1) / th);
-    const bool fp16 = g.storage == PF_STORE_FP16;
-    const long target = fp16 ? 8192 : 32768;
+    // the binary16 and moment forms march long chunks (the moment form
+    // stages two sending planes more than it updates per chunk)
+    const bool fp16 = g.storage == PF_STORE_FP16 || g.storage == PF_STORE_MOM;
+    const long target = g.storage == PF_STORE_MOM ? 16384 : (fp16 ? 8192 : 32768);
     long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;
@@ -259,7 +268,11 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     const pf_config& g = *cfg;
     if (g.velocity_set != PF_D3Q19 && g.velocity_set != PF_D3Q27) return PF_ERR_CONFIG;
     if (g.flow_case != PF_JET && g.flow_case != PF_DROPLET) return PF_ERR_CONFIG;
-    if (g.storage != PF_STORE_FP32 && g.storage != PF_STORE_FP16 && g.storage != PF_STORE_FP16_NE) return PF_ERR_CONFIG;
+    if (g.storage != PF_STORE_FP32 && g.storage != PF_STORE_FP16 && g.storage != PF_STORE_FP16_NE &&
+        g.storage != PF_STORE_MOM)
+        return PF_ERR_CONFIG;
+    if (g.storage == PF_STORE_MOM && (g.kernel == PF_KERNEL_LITERAL || g.closure == PF_CLOSURE_MLP))
+        return PF_ERR_CONFIG;
     if (g.storage == PF_STORE_FP16_NE) {
         // pair collision only; the nozzle must lie inside the lateral ghost
         // shell, where the boundary kernel's source columns need no wrap
@@ -288,8 +301,9 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     c->N = N;
     c->Q = g.velocity_set;
     c->jet = g.flow_case == PF_JET;
-    c->half = g.storage != PF_STORE_FP32;
+    c->half = g.storage == PF_STORE_FP16 || g.storage == PF_STORE_FP16_NE;
     c->ne = g.storage == PF_STORE_FP16_NE;
+    c->moments = g.storage == PF_STORE_MOM;
     // Row-interleaved population layout: row pitch S*nx plus `slot_pad`
     // elements (default none), plane pitch = row pitch * ny.  In-plane byte
     // offsets are 32-bit in the kernels.
@@ -335,7 +349,12 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
         c->bytes += bytes;
         return true;
     };
-    bool ok = alloc(&c->f, c->f_elems * pop_bytes(c)) && alloc(&c->h, 3 * N * pop_bytes(c));
+    bool ok = (c->moments || alloc(&c->f, c->f_elems * pop_bytes(c))) && alloc(&c->h, 3 * N * pop_bytes(c));
+    if (c->moments) {
+        const size_t nc = c->Q == 27 ? 17 : 10;
+        ok = ok && alloc((void**)&c->rec[0], nc * N * sizeof(float)) && alloc((void**)&c->rec[1], nc * N * sizeof(float));
+        ok = ok && alloc((void**)&c->bcf, 18 * (size_t)g.nx * g.ny * sizeof(float));
+    }
     for (int i = 0; ok && i < pf::kNumFields; ++i) ok = alloc((void**)&c->fld[i], N * sizeof(float));
     ok = ok && alloc((void**)&c->phi_next, N * sizeof(float));
     const size_t plane = (size_t)g.nx * g.ny;
@@ -371,6 +390,10 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     k.ne_kinv = 1.0f / pf::kNeK;
     k.mom_prev = c->mom[0];
     k.mom_cur = c->mom[1];
+    k.mom = c->moments ? 1 : 0;
+    k.rec_prev = c->rec[0];
+    k.rec_cur = c->rec[1];
+    k.bcf = c->bcf;
     k.nx = g.nx;
     k.ny = g.ny;
     k.nz = g.nz;
@@ -381,7 +404,7 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     k.zchunk = choose_zchunk(g);
     k.tile_x = tile_x(g);
     k.ntx = (g.nx + k.tile_x - 1) / k.tile_x;
-    k.nty = (g.ny + pf::kThreads / k.tile_x - 1) / (pf::kThreads / k.tile_x);
+    k.nty = (g.ny + tile_rows(g, k.tile_x) - 1) / tile_rows(g, k.tile_x);
     k.ntz = (g.nz - 2 + k.zchunk - 1) / k.zchunk;
     fill_constants(c);
     *out = c;
@@ -403,6 +426,8 @@ void pf_destroy(pf_ctx* c) {
     (void)hipFree(c->gather_buf);
     (void)hipFree(c->kappa);
     for (auto* p : c->mom) (void)hipFree(p);
+    for (auto* p : c->rec) (void)hipFree(p);
+    (void)hipFree(c->bcf);
     for (auto* p : c->der) (void)hipFree(p);
     (void)hipFree(c->mlp_w);
     (void)hipFree(c->mlp_b);
@@ -421,6 +446,7 @@ int pf_init_distros(pf_ctx* c) {
     const size_t plane = (size_t)c->cfg.nx * c->cfg.ny;
     PF_HIP(c, hipMemsetAsync(c->bcp, 0, 2 * pf::kPlaneVals * plane * sizeof(float), c->stream));
     PF_HIP(c, hipMemsetAsync(c->gbc, 0, 2 * plane * sizeof(float), c->stream));
+    if (c->bcf) PF_HIP(c, hipMemsetAsync(c->bcf, 0, 18 * plane * sizeof(float), c->stream));
     PF_HIP(c, hipMemsetAsync(c->h, 0, 3 * c->N * pop_bytes(c), c->stream));
     // the ghost shell of phi is read from whichever buffer is current
     PF_HIP(c, hipMemcpyAsync(c->phi_next, c->fld[PF_FIELD_PHI], c->N * sizeof(float), hipMemcpyDeviceToDevice,
@@ -439,7 +465,7 @@ int pf_init(pf_ctx* c) {
         PF_HIP(c, hipMemsetAsync(c->fld[i], 0, c->N * sizeof(float), c->stream));
     for (auto* d : c->der)
         if (d) PF_HIP(c, hipMemsetAsync(d, 0, c->N * sizeof(float), c->stream));
-    PF_HIP(c, hipMemsetAsync(c->f, 0, c->f_elems * pop_bytes(c), c->stream));
+    if (c->f) PF_HIP(c, hipMemsetAsync(c->f, 0, c->f_elems * pop_bytes(c), c->stream));
     PF_HIP(c, pf::launch_init_fields(c->kp, c->jet, c->stream));
     if (!c->jet) {
         // setDroplet (src/initialConditions.cu:58-83), evaluated on the host
@@ -495,6 +521,11 @@ static void swap_phi(pf_ctx* c) {
 // Non-equilibrium form: the M this step wrote become those the next step
 // reads.
 static void swap_mom(pf_ctx* c) {
+    if (c->moments) {  // moment form: likewise the coefficients
+        std::swap(c->rec[0], c->rec[1]);
+        c->kp.rec_prev = c->rec[0];
+        c->kp.rec_cur = c->rec[1];
+    }
     if (!c->ne) return;
     std::swap(c->mom[0], c->mom[1]);
     c->kp.mom_prev = c->mom[0];
@@ -505,12 +536,16 @@ static void swap_mom(pf_ctx* c) {
 // learned closure when on, and the collide-and-stream of this parity.
 static int launch_core(pf_ctx* c, int variant, hipEvent_t* ev) {
     if (ev) PF_HIP(c, hipEventRecord(ev[0], c->stream));
+    const bool local = c->fresh;
     PF_HIP(c, pf::launch_phase_sum(c->kp, c->jet, c->fresh, c->half, c->stream));
     swap_phi(c);
     c->fresh = false;
     if (variant == 2) PF_HIP(c, pf::launch_closure(c->kp, c->stream));
     if (ev) PF_HIP(c, hipEventRecord(ev[1], c->stream));
-    PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
+    if (c->moments)
+        PF_HIP(c, pf::launch_step_mom(c->kp, c->Q, c->jet, local, c->stream));
+    else
+        PF_HIP(c, pf::launch_step(c->kp, c->Q, c->jet, c->half, c->parity, variant, c->stream));
     if (ev) PF_HIP(c, hipEventRecord(ev[2], c->stream));
     return PF_OK;
 }
@@ -575,7 +610,10 @@ static int run_steps(pf_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t flags, hi
             const int rc = launch_core(c, variant, ev);
             if (rc != PF_OK) return rc;
         }
-        if (c->jet) PF_HIP(c, pf::launch_jet_bc(c->kp, c->Q, c->half, c->parity ^ 1, c->stream));
+        if (c->jet && c->moments)
+            PF_HIP(c, pf::launch_jet_bc_mom(c->kp, c->Q, c->stream));
+        else if (c->jet)
+            PF_HIP(c, pf::launch_jet_bc(c->kp, c->Q, c->half, c->parity ^ 1, c->stream));
         swap_mom(c);
         if (avg) PF_HIP(c, pf::launch_average(c->kp, dp, (float)(t0 + s + 1), c->stream));
         if (ev) PF_HIP(c, hipEventRecord(ev[3], c->stream));

@@ -8,6 +8,10 @@ hipError_t launch_step_q19(const KParams& p, bool jet, bool half, int parity, in
 hipError_t launch_step_q27(const KParams& p, bool jet, bool half, int parity, int variant, hipStream_t s);
 hipError_t launch_jet_bc_q19(const KParams& p, bool half, int next_parity, hipStream_t s);
 hipError_t launch_jet_bc_q27(const KParams& p, bool half, int next_parity, hipStream_t s);
+hipError_t launch_step_mom_q19(const KParams& p, bool jet, bool local, hipStream_t s);
+hipError_t launch_step_mom_q27(const KParams& p, bool jet, bool local, hipStream_t s);
+hipError_t launch_jet_bc_mom_q19(const KParams& p, hipStream_t s);
+hipError_t launch_jet_bc_mom_q27(const KParams& p, hipStream_t s);
 
 template <bool HALF>
 static hipError_t phase_sum_h(const KParams& p, bool jet, bool local, hipStream_t s) {
@@ -51,6 +55,14 @@ hipError_t launch_step(const KParams& p, int Q, bool jet, bool half, int parity,
 
 hipError_t launch_jet_bc(const KParams& p, int Q, bool half, int next_parity, hipStream_t s) {
     return Q == 27 ? launch_jet_bc_q27(p, half, next_parity, s) : launch_jet_bc_q19(p, half, next_parity, s);
+}
+
+hipError_t launch_step_mom(const KParams& p, int Q, bool jet, bool local, hipStream_t s) {
+    return Q == 27 ? launch_step_mom_q27(p, jet, local, s) : launch_step_mom_q19(p, jet, local, s);
+}
+
+hipError_t launch_jet_bc_mom(const KParams& p, int Q, hipStream_t s) {
+    return Q == 27 ? launch_jet_bc_mom_q27(p, s) : launch_jet_bc_mom_q19(p, s);
 }
 
 }  // namespace pf

@@ -1000,7 +1000,9 @@ __device__ __forceinline__ bool in_nozzle(const KParams& p, int x, int y) {
     return dx * dx + dy * dy <= p.R2;
 }
 
-template <int Q, bool HALF, int NPAR, bool NE = false>
+// MOM: the moment form, which rebuilds populations from coefficients: the
+// boundary populations go to p.bcf (kparams.hpp) instead of f.
+template <int Q, bool HALF, int NPAR, bool NE = false, bool MOM = false>
 __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
 #pragma clang fp contract(off)
     const int x = 1 + blockIdx.x * kTX + (threadIdx.x & (kTX - 1));
@@ -1052,10 +1054,14 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                 }
                 if constexpr (NE)
                     val = p.ne_k * (val - ne_shift<Q, q>(inflow_m(xw3[1 - kCX[q]], yw3[1 - kCY[q]]).m));
-                int slot;
-                Loc l;
-                in_slot<q, NPAR>(nb, slot, l);
-                stf<HALF>(p, slot, l, val);
+                if constexpr (MOM) {
+                    p.bcf[(uint32_t)((kCY[q] + 1) * 3 + kCX[q] + 1) * plane + pi] = val;
+                } else {
+                    int slot;
+                    Loc l;
+                    in_slot<q, NPAR>(nb, slot, l);
+                    stf<HALF>(p, slot, l, val);
+                }
             }
         });
         float g5 = 0.0f;
@@ -1105,10 +1111,14 @@ __global__ __launch_bounds__(kThreads) void k_jet_bc(KParams p) {
                             omco_zmax * f_neq<Q, q>(pxx, pyy, pzz, pxy, pxz, pyz, tux, tuy, tuz) +
                             0.5f * force_of<Q, q>(cu, ux, uy, uz, tfx, tfy, tfz);
                 if constexpr (NE) val = p.ne_k * (val - ne_shift<Q, q>(ne_mom(rho_b, ux, uy, uz).m));
-                int slot;
-                Loc l;
-                in_slot<q, NPAR>(nb, slot, l);
-                stf<HALF>(p, slot, l, val);
+                if constexpr (MOM) {
+                    p.bcf[(uint32_t)(9 + (kCY[q] + 1) * 3 + kCX[q] + 1) * plane + pi] = val;
+                } else {
+                    int slot;
+                    Loc l;
+                    in_slot<q, NPAR>(nb, slot, l);
+                    stf<HALF>(p, slot, l, val);
+                }
             }
         });
         const float phi_b = phi_g[c];

@@ -1,0 +1,371 @@
+// k_step_mom: one timestep of the moment storage form (mom.hpp).
+//
+// The same step as k_step (src/phaseField.cuh:54-219, src/lbm.cu:33-239),
+// with the populations kept as the coefficients of their senders' collision
+// output instead of stored one by one.  A workgroup marches a TX x TY column
+// of cells through its z-chunk.  Per plane:
+//   - the phi ring and the normals of k_step (same LDS layout);
+//   - the coefficients of one sending plane w, with a 1-cell halo, are staged
+//     in LDS, and each thread rebuilds from them the populations its column
+//     receives from plane w: those with c_z = +1 belong to its cell in plane
+//     w + 1, c_z = 0 to plane w, c_z = -1 to plane w - 1.  It sums each c_z
+//     layer into the raw moments (sum f, sum f c, sum f c c) of the cell it
+//     arrives at, so it carries the partial moments of three cells;
+//   - the cell of plane w - 1 then has all its arrivals (from planes w - 2,
+//     w - 1, w: always in that order, so the sums do not depend on the tile
+//     or the z-chunk): its collision writes its own coefficients, the phase
+//     flux, and on save steps the macroscopic fields.
+// The jet's ghost planes send the boundary populations of k_jet_bc (p.bcf)
+// instead.  The first step after initialisation (local) sums each cell's
+// own coefficients, the equilibrium setDistros leaves in every slot
+// (src/initialConditions.cu:101-138).
+#pragma once
+
+#include "mom.hpp"
+#include "step_kernels.hpp"
+
+namespace pf {
+
+// Tiles of TX x 4 cells (TX * 4 threads): 64 x 4 by default, whose LDS
+// (56.8 KB) lets two workgroups share a CU.
+constexpr int kMomTY = kMomRows;
+
+template <int Q, bool JET, bool SAVE, int TX>
+__global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) {
+    constexpr int NC = mom_nc<Q>();
+    constexpr int TY = kMomTY;
+    constexpr int kThreads = TX * TY;
+    constexpr int HX = TX + 4, HY = TY + 4;    // phi tile with 2-cell halo
+    constexpr int NX1 = TX + 2, NY1 = TY + 2;  // normal and coefficient tiles with 1-cell halo
+    constexpr int kRing = 5;
+    constexpr int kRec = NX1 * NY1;
+    __shared__ float s_phi[kRing][HY * HX];
+    __shared__ float s_n[3][4][NY1 * NX1];
+    __shared__ float s_rec[NC][kRec];
+
+    const int tid = threadIdx.x;
+    const int tx = tid & (TX - 1), ty = tid / TX;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x, (uint32_t)p.ntx);
+    const int bx = (int)(lb % (uint32_t)p.ntx);
+    const uint32_t rem = lb / (uint32_t)p.ntx;
+    const int by = (int)(rem % (uint32_t)p.nty);
+    const int bz = (int)(rem / (uint32_t)p.nty);
+    const int x0 = bx * TX, y0 = by * TY;
+    const int zs = 1 + bz * p.zchunk;
+    const int ze = min(zs + p.zchunk, nz - 1);
+    const float* __restrict__ phi_g = p.fld[10];
+    const uint32_t plane = (uint32_t)nx * (uint32_t)ny;
+
+    // phi of plane pz, tile + 2-cell halo, zero outside the grid (as k_step)
+    constexpr int kPhiIt = (HX * HY + kThreads - 1) / kThreads;
+    auto fetch_phi = [&](int pz, float (&v)[kPhiIt]) {
+        const int cz = min(max(pz, 0), nz - 1);
+#pragma unroll
+        for (int j = 0; j < kPhiIt; ++j) {
+            const int i = min(tid + j * kThreads, HX * HY - 1);
+            const int gx = min(max(x0 - 2 + i % HX, 0), nx - 1), gy = min(max(y0 - 2 + i / HX, 0), ny - 1);
+            v[j] = phi_g[(uint32_t)gx + (uint32_t)nx * ((uint32_t)gy + (uint32_t)ny * (uint32_t)cz)];
+        }
+    };
+    auto put_phi = [&](int pz, const float (&v)[kPhiIt]) {
+        float* dst = s_phi[(pz + kRing) % kRing];
+        const bool zin = pz >= 0 && pz < nz;
+#pragma unroll
+        for (int j = 0; j < kPhiIt; ++j) {
+            const int i = tid + j * kThreads;
+            const int gx = x0 - 2 + i % HX, gy = y0 - 2 + i / HX;
+            if (i < HX * HY) dst[i] = (zin && gx >= 0 && gx < nx && gy >= 0 && gy < ny) ? v[j] : 0.0f;
+        }
+    };
+
+    // Coefficients of sending plane pw over the 1-halo tile, taken at the
+    // ghost-shell-wrapped coordinates of the cells that sent them (x and y;
+    // z for the droplet).  The jet's ghost planes are never read from here
+    // (their populations come from p.bcf); the load is clamped into the
+    // interior.
+    constexpr int kRIt = (kRec + kThreads - 1) / kThreads;
+    auto fetch_rec = [&](int pw, float (&v)[kRIt][NC]) {
+        int cz = pw;
+        if constexpr (!JET) cz = (pw < 1) ? nz - 2 : (pw > nz - 2 ? 1 : pw);
+        cz = min(max(cz, 1), nz - 2);
+#pragma unroll
+        for (int j = 0; j < kRIt; ++j) {
+            const int i = min(tid + j * kThreads, kRec - 1);
+            const int gx = x0 - 1 + i % NX1, gy = y0 - 1 + i / NX1;
+            const int wx = gx <= 0 ? nx - 2 : (gx >= nx - 1 ? 1 : gx);
+            const int wy = gy <= 0 ? ny - 2 : (gy >= ny - 1 ? 1 : gy);
+            const uint32_t c = (uint32_t)wx + (uint32_t)nx * ((uint32_t)wy + (uint32_t)ny * (uint32_t)cz);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v[j][k] = p.rec_prev[(size_t)k * p.N + c];
+        }
+    };
+    auto put_rec = [&](const float (&v)[kRIt][NC]) {
+#pragma unroll
+        for (int j = 0; j < kRIt; ++j) {
+            const int i = tid + j * kThreads;
+            if (i < kRec) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) s_rec[k][i] = v[j][k];
+            }
+        }
+    };
+
+    // normals and |grad phi| on the 1-halo tile of plane pz
+    // (src/phaseField.cuh:54-140), as k_step
+    auto comp_normals = [&](int pz) {
+        const int r = pz % 3;
+        const float* pm = s_phi[(pz + kRing - 1) % kRing];
+        const float* p0 = s_phi[pz % kRing];
+        const float* pp = s_phi[(pz + 1) % kRing];
+        for (int i = tid; i < NX1 * NY1; i += kThreads) {
+            const int lx = i % NX1, ly = i / NX1;
+            const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+            float nrx = 0.0f, nry = 0.0f, nrz = 0.0f, ind = 0.0f;
+            if (gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2 && pz >= 1 && pz <= nz - 2) {
+                const int o = (ly + 1) * HX + (lx + 1);
+                auto val = [&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    constexpr int off = kCY[q] * HX + kCX[q];
+                    const float* pl = kCZ[q] < 0 ? pm : (kCZ[q] > 0 ? pp : p0);
+                    return pl[o + off];
+                };
+                const float gx_ = kAs2 * iso_sum<Q, 0>(val);
+                const float gy_ = kAs2 * iso_sum<Q, 1>(val);
+                const float gz_ = kAs2 * iso_sum<Q, 2>(val);
+                ind = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_);
+                const float inv = 1.0f / (ind + 1e-9f);
+                nrx = gx_ * inv;
+                nry = gy_ * inv;
+                nrz = gz_ * inv;
+            }
+            s_n[r][0][i] = nrx;
+            s_n[r][1][i] = nry;
+            s_n[r][2][i] = nrz;
+            s_n[r][3][i] = ind;
+        }
+    };
+
+    // Rebuild the populations sent by plane pw to this thread's column and
+    // add each c_z layer to the raw moments of the cell it reaches: am (plane
+    // pw - 1), a0 (pw), ap (pw + 1).  DM / D0 / DP: which layers are wanted.
+    const int o1 = (ty + 1) * NX1 + (tx + 1);
+    const uint32_t pcol = (uint32_t)min(x0 + tx, nx - 1) + (uint32_t)nx * (uint32_t)min(y0 + ty, ny - 1);
+    auto bc_layer = [&](int base) {
+        MomLayer g{0, 0, 0, 0, 0, 0, false};
+        static_for<9>([&](auto oc) {
+            constexpr int o = decltype(oc)::value, cx = o % 3 - 1, cy = o / 3 - 1;
+            if constexpr (mom_has<Q>(cx, cy, 1)) layer_add<cx, cy>(g, p.bcf[(uint32_t)(base + o) * plane + pcol]);
+        });
+        return g;
+    };
+    auto scatter = [&](auto dm_, auto d0_, auto dp_, int pw, MomAcc& am, MomAcc& a0, MomAcc& ap) {
+        constexpr bool DM = decltype(dm_)::value, D0 = decltype(d0_)::value, DP = decltype(dp_)::value;
+        if constexpr (JET) {
+            if (pw == 0) {  // nozzle inflow and frozen wall entering z = 1
+                if constexpr (DP) layer_to<1>(ap, bc_layer(0));
+                return;
+            }
+            if (pw == nz - 1) {  // outflow entering z = nz-2
+                if constexpr (DM) layer_to<-1>(am, bc_layer(9));
+                return;
+            }
+        }
+        MomLayer gm{0, 0, 0, 0, 0, 0, false}, g0 = gm, gp = gm;
+        static_for<9>([&](auto oc) {
+            constexpr int o = decltype(oc)::value, cx = o % 3 - 1, cy = o / 3 - 1;
+            constexpr bool hz = mom_has<Q>(cx, cy, 1);
+            const int src = o1 - cy * NX1 - cx;
+            auto r = [&](int k) { return s_rec[k][src]; };
+            float fm = 0.0f, f0 = 0.0f, fp = 0.0f;
+            mom_column<Q, cx, cy, DM && hz, D0, DP && hz>(r, fm, f0, fp);
+            if constexpr (DM && hz) layer_add<cx, cy>(gm, fm);
+            if constexpr (D0) layer_add<cx, cy>(g0, f0);
+            if constexpr (DP && hz) layer_add<cx, cy>(gp, fp);
+        });
+        if constexpr (DP) layer_to<1>(ap, gp);
+        if constexpr (D0) layer_to<0>(a0, g0);
+        if constexpr (DM) layer_to<-1>(am, gm);
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+
+    // prologue: phi planes zs-2 .. zs+1, normals of zs-1 and zs, and the
+    // arrivals of plane zs from zs-1 and zs (and of zs+1 from zs)
+    {
+        float v[4][kPhiIt];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) fetch_phi(zs - 2 + k, v[k]);
+        float rv[kRIt][NC];
+        if (!local) fetch_rec(zs - 1, rv);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) put_phi(zs - 2 + k, v[k]);
+        if (!local) put_rec(rv);
+    }
+    lds_barrier();
+    comp_normals(zs - 1);
+    comp_normals(zs);
+    MomAcc a_lo = mom_zero(), a_mid = mom_zero(), a_hi = mom_zero(), a_off = mom_zero();
+    if (!local) {
+        scatter(F{}, F{}, T{}, zs - 1, a_off, a_off, a_lo);
+        lds_barrier();
+        float rv[kRIt][NC];
+        fetch_rec(zs, rv);
+        put_rec(rv);
+        lds_barrier();
+        scatter(F{}, T{}, T{}, zs, a_off, a_lo, a_mid);
+    }
+    lds_barrier();
+
+    const int x = x0 + tx, y = y0 + ty;
+    const bool active = (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2);
+
+    // The coefficients of sending plane z + 1 and phi of plane z + 2 are
+    // loaded one iteration ahead: their loads are in flight during the
+    // previous plane's work.
+    float rv[kRIt][NC];
+    float phv[kPhiIt];
+    if (!local) fetch_rec(zs + 1, rv);
+    fetch_phi(zs + 2, phv);
+    for (int z = zs; z < ze; ++z) {
+        put_phi(z + 2, phv);
+        if (!local) put_rec(rv);
+        lds_barrier();
+        if (z + 1 < ze) {
+            if (!local) fetch_rec(z + 2, rv);
+            fetch_phi(z + 3, phv);
+        }
+        comp_normals(z + 1);
+        if (!local) scatter(T{}, T{}, T{}, z + 1, a_lo, a_mid, a_hi);
+        lds_barrier();
+
+        // the arrivals of plane z are complete
+        MomAcc a = a_lo;
+        a_lo = a_mid;
+        a_mid = a_hi;
+        a_hi = mom_zero();
+
+        // ---- curvature and continuum surface force (src/phaseField.cuh:142-219)
+        const int o = (ty + 1) * NX1 + (tx + 1);
+        const float* nm[3] = {s_n[(z + 2) % 3][0], s_n[z % 3][0], s_n[(z + 1) % 3][0]};
+        auto nval = [&](int comp_, auto qc) {
+            constexpr int q = decltype(qc)::value;
+            constexpr int off = kCY[q] * NX1 + kCX[q];
+            return nm[kCZ[q] + 1][comp_ * (NY1 * NX1) + o + off];
+        };
+        const float phi = s_phi[z % kRing][(ty + 2) * HX + (tx + 2)];
+        if (!active) continue;
+        const float scx = iso_sum<Q, 0>([&](auto qc) { return nval(0, qc); });
+        const float scy = iso_sum<Q, 1>([&](auto qc) { return nval(1, qc); });
+        const float scz = iso_sum<Q, 2>([&](auto qc) { return nval(2, qc); });
+        const float curvature = kAs2 * (scx + scy + scz);
+        const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
+        const float ind = s_n[z % 3][3][o];
+        const float nrx = s_n[z % 3][0][o], nry = s_n[z % 3][1][o], nrz = s_n[z % 3][2][o];
+        const float stCurv = -p.sigma * curvature * ind;
+        const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
+
+        // ---- relaxation (src/lbm.cu:158-172)
+        float omega;
+        if constexpr (JET) {
+            const float tau_phi = (1.0f - phi) * p.tau_water + phi * p.tau_oil;
+            const float zn = (float)z * p.sp_inv_nz_m1;
+            float s = (zn - p.sp_z_start) * p.sp_inv_sponge;
+            s = fminf(fmaxf(s, 0.0f), 1.0f);
+            const float r = s * s * (3.0f - 2.0f * s);
+            const float visc_local = (1.0f - phi) * p.visc_water + phi * p.visc_oil;
+            const float tau_zmax = 0.5f + kAs2 * (visc_local * (1.0f + 100.0f));
+            const float tau_eff = tau_phi + r * (tau_zmax - tau_phi);
+            omega = 1.0f / tau_eff;
+        } else {
+            omega = p.omega_ref;
+        }
+        const float omco = 1.0f - omega;
+
+        if (local) {
+            float own[NC];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) own[k] = p.rec_prev[(size_t)k * p.N + c];
+            a = mom_local<Q>([&](int k) { return own[k]; });
+        }
+
+        // ---- moments (src/lbm.cu:49-120): rho = 1 + sum f~, u with half the
+        // force, deviatoric non-equilibrium stress
+        const float m0 = a.m0;
+        const float rho = m0 + 1.0f;
+        const float invRho = 1.0f / rho;
+        const float ux = (a.jx + 0.5f * fsx) * invRho;
+        const float uy = (a.jy + 0.5f * fsy) * invRho;
+        const float uz = (a.jz + 0.5f * fsz) * invRho;
+        const float axx = a.xx - rho * ux * ux + fsx * ux;
+        const float ayy = a.yy - rho * uy * uy + fsy * uy;
+        const float azz = a.zz - rho * uz * uz + fsz * uz;
+        const float pxy = a.xy - rho * ux * uy + 0.5f * (fsx * uy + fsy * ux);
+        const float pxz = a.xz - rho * ux * uz + 0.5f * (fsx * uz + fsz * ux);
+        const float pyz = a.yz - rho * uy * uz + 0.5f * (fsy * uz + fsz * uy);
+        const float tr3 = (axx + ayy + azz) * kCs2;
+        const float pxx = axx - tr3, pyy = ayy - tr3, pzz = azz - tr3;
+
+        // ---- collision: this cell's coefficients (mom.hpp)
+        {
+            float cf[NC];
+            mom_coeffs<Q>(m0, rho, ux, uy, uz, fsx, fsy, fsz, pxx, pyy, pzz, pxy, pxz, pyz, omco, cf);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(cf[k], p.rec_cur + (size_t)k * p.N + c);
+        }
+
+        // ---- phase flux (src/lbm.cu:209-238, phase_flux.hpp)
+        {
+            const float sharp = p.gamma * phi * (1.0f - phi);
+            const float p4 = 4.0f * phi;
+            __builtin_nontemporal_store(0.125f * (p4 * ux + sharp * nrx), static_cast<float*>(p.h[0]) + c);
+            __builtin_nontemporal_store(0.125f * (p4 * uy + sharp * nry), static_cast<float*>(p.h[1]) + c);
+            __builtin_nontemporal_store(0.125f * (p4 * uz + sharp * nrz), static_cast<float*>(p.h[2]) + c);
+        }
+
+        // ---- macroscopic fields for output and for the jet boundary planes
+        if constexpr (SAVE) {
+            float* const* Fd = p.fld;
+            Fd[0][c] = rho; Fd[1][c] = ux; Fd[2][c] = uy; Fd[3][c] = uz;
+            Fd[4][c] = pxx; Fd[5][c] = pyy; Fd[6][c] = pzz; Fd[7][c] = pxy; Fd[8][c] = pxz; Fd[9][c] = pyz;
+            Fd[11][c] = nrx; Fd[12][c] = nry; Fd[13][c] = nrz; Fd[14][c] = ind;
+            Fd[15][c] = fsx; Fd[16][c] = fsy; Fd[17][c] = fsz;
+        }
+        if constexpr (JET) {
+            if (z == 1 || z == nz - 2) {
+                float* b = p.bcp + (z == 1 ? 0u : (uint32_t)kPlaneVals * plane) + (uint32_t)x + (uint32_t)nx * (uint32_t)y;
+                const float v[kPlaneVals] = {rho, ux, uy, uz, pxx, pyy, pzz, pxy, pxz, pyz, fsx, fsy, fsz};
+#pragma unroll
+                for (int kk = 0; kk < kPlaneVals; ++kk) b[(uint32_t)kk * plane] = v[kk];
+            }
+        }
+    }
+}
+
+template <int Q, bool JET, bool SAVE>
+static hipError_t step_mom_t(const KParams& p, bool local, hipStream_t s) {
+    const uint32_t nblk = (uint32_t)p.ntx * (uint32_t)p.nty * (uint32_t)p.ntz;
+    if (p.tile_x == 128)
+        hipLaunchKernelGGL((k_step_mom<Q, JET, SAVE, 128>), dim3(nblk), dim3(128 * kMomTY), 0, s, p, local ? 1 : 0);
+    else if (p.tile_x == 64)
+        hipLaunchKernelGGL((k_step_mom<Q, JET, SAVE, 64>), dim3(nblk), dim3(64 * kMomTY), 0, s, p, local ? 1 : 0);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <int Q>
+static hipError_t step_mom_q(const KParams& p, bool jet, bool local, hipStream_t s) {
+    const bool save = p.save != 0;
+    if (jet) return save ? step_mom_t<Q, true, true>(p, local, s) : step_mom_t<Q, true, false>(p, local, s);
+    return save ? step_mom_t<Q, false, true>(p, local, s) : step_mom_t<Q, false, false>(p, local, s);
+}
+
+template <int Q>
+static hipError_t jet_bc_mom_q(const KParams& p, hipStream_t s) {
+    dim3 grid((p.nx - 2 + kTX - 1) / kTX, (p.ny - 2 + kTY - 1) / kTY, 1);
+    hipLaunchKernelGGL((k_jet_bc<Q, false, 0, false, true>), grid, dim3(kThreads), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace pf

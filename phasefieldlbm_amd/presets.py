@@ -9,7 +9,7 @@ Relations kept from the reference: nu = u*D/Re, sigma = u^2*D/We
 from __future__ import annotations
 
 from . import (CLOSURE_FD, D3Q19, D3Q27, DROPLET, JET, NOISE_GAUSSIAN_XY, NOISE_UNIFORM_Z, STORE_FP16,
-               STORE_FP16_NE, STORE_FP32, Config)
+               STORE_FP16_NE, STORE_FP32, STORE_MOM, Config)
 
 
 def reference_jet(velocity_set: int = D3Q19) -> Config:
@@ -66,6 +66,8 @@ WORKLOADS = {
     "jet_d3q27_128x128x512_fp16": (lambda: jet_d3q27_128x128x512(STORE_FP16), 5000),
     "jet_d3q27_256x256x1024_fp16_ne": (lambda: jet_d3q27_256x256x1024(STORE_FP16_NE), 20000),
     "jet_d3q27_128x128x512_fp16_ne": (lambda: jet_d3q27_128x128x512(STORE_FP16_NE), 5000),
+    "jet_d3q27_256x256x1024_mom": (lambda: jet_d3q27_256x256x1024(STORE_MOM), 20000),
+    "jet_d3q27_128x128x512_mom": (lambda: jet_d3q27_128x128x512(STORE_MOM), 5000),
     # the 128x128x512 accuracy case at a quarter of the resolution (D = 8)
     "jet_d3q27_40x40x128": (lambda: round_jet(40, 40, 128, 8, D3Q27), 5000),
 }
@@ -75,7 +77,7 @@ def bytes_per_update(cfg: Config) -> int:
     """Algorithmic HBM bytes of one cell update in the reference's scheme:
     read and write every f and g population once, write and read phi once
     (f and g in the storage type)."""
-    s = 2 if cfg.storage != STORE_FP32 else 4
+    s = 2 if cfg.storage in (STORE_FP16, STORE_FP16_NE) else 4
     return 2 * cfg.velocity_set * s + 2 * 7 * s + 8
 
 
@@ -85,7 +87,12 @@ def bytes_moved_per_update(cfg: Config) -> int:
     phi and writes the 3-component phase flux, the phase sum reads phi and
     the flux and writes phi (2 Q s + 3 s + 4 + 3 s + 8; f and the flux in
     the storage type, phi in FP32); the non-equilibrium form also writes and
-    reads the 8-byte binary16 moments of every cell."""
+    reads the 8-byte binary16 moments of every cell; the moment form reads
+    and writes 10 (D3Q19) or 17 (D3Q27) FP32 coefficients per cell instead of
+    the Q populations."""
+    if cfg.storage == STORE_MOM:
+        nc = 17 if cfg.velocity_set == 27 else 10
+        return 2 * nc * 4 + 6 * 4 + 12
     s = 2 if cfg.storage != STORE_FP32 else 4
     extra = 16 if cfg.storage == STORE_FP16_NE else 0  # binary16 moments written and read
     return 2 * cfg.velocity_set * s + 6 * s + 12 + extra
@@ -93,4 +100,4 @@ def bytes_moved_per_update(cfg: Config) -> int:
 
 __all__ = ["reference_jet", "reference_droplet", "static_droplet", "round_jet", "jet_d3q19_128x128x512",
            "jet_d3q27_256x256x1024", "jet_d3q27_128x128x512", "WORKLOADS", "bytes_per_update", "bytes_moved_per_update",
-           "STORE_FP32", "STORE_FP16", "STORE_FP16_NE"]
+           "STORE_FP32", "STORE_FP16", "STORE_FP16_NE", "STORE_MOM"]

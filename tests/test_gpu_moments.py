@@ -1,0 +1,144 @@
+"""The moment storage form (PF_STORE_MOM) through the C-ABI.
+
+Each cell stores the FP32 coefficients of the polynomial in c that its
+regularized collision output is, and the next step rebuilds every arriving
+population from its sender's coefficients (phasefieldlbm_amd/csrc/mom.hpp).
+It is the reference's scheme (src/lbm.cu:33-239) with FP32 storage and FP32
+arithmetic, evaluated in a different order, so it is held to the FP32 bars
+of test_gpu_parity.py against the FP32 oracle: 1e-5 relative L-infinity on
+every field (each of its own maximum), with the FP64 envelope fallback for
+the fields that are small against their own round-off.  Rebuilt
+populations are compared in the reference layout, and the accuracy against
+FP64 on the reduced 40x40x128 jet fixture is held to the FP32 bound
+(1e-3 at every snapshot to 5000 steps).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import phasefieldlbm_amd as pf
+from phasefieldlbm_amd import D3Q19, D3Q27, NOISE_GAUSSIAN_XY, NOISE_NONE, NOISE_UNIFORM_Z, STORE_MOM, presets
+import oracle
+
+from test_gpu_parity import CORE, I, compare, pop_scale
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+def test_mom_droplet(Q):
+    compare(presets.static_droplet(n=32, radius=8.0, velocity_set=Q, storage=STORE_MOM), 200)
+
+
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+def test_mom_droplet_all_fields(Q):
+    compare(presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=STORE_MOM), 40, fields=pf.FIELDS)
+
+
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+@pytest.mark.parametrize("noise", [NOISE_UNIFORM_Z, NOISE_NONE])
+def test_mom_jet(Q, noise):
+    """Jet with nozzle inflow, frozen wall, outflow and sponge: every field,
+    ghost planes included, after 60 steps (the jet crosses the outflow)."""
+    cfg = presets.round_jet(20, 20, 40, 6, Q, storage=STORE_MOM, noise=noise)
+    compare(cfg, 60, fields=pf.FIELDS, region=(slice(None),) * 3)
+
+
+def test_mom_reference_jet_gaussian():
+    """The reference's default jet (64x64x128, D = 10, Gaussian inflow noise
+    through logf/cosf, whose last-ulp behaviour differs between the GPU and
+    the host C library): 1e-4."""
+    cfg = presets.reference_jet(D3Q27)
+    cfg.storage = STORE_MOM
+    compare(cfg, 40, tol=1e-4)
+
+
+@pytest.mark.parametrize("steps", [0, 1, 4, 5])
+@pytest.mark.parametrize("case", ["droplet", "jet"])
+def test_mom_populations(steps, case):
+    """The populations the next step reads, rebuilt from the coefficients in
+    the reference layout: right after initialisation (each cell's own
+    equilibrium), after the first step and after later steps (the jet's
+    boundary populations included)."""
+    cfg = (presets.static_droplet(n=16, radius=4.0, velocity_set=D3Q27, storage=STORE_MOM) if case == "droplet"
+           else presets.round_jet(16, 16, 32, 6, D3Q27, storage=STORE_MOM, noise=NOISE_UNIFORM_Z))
+    with pf.Simulation(cfg) as sim:
+        sim.init()
+        if steps:
+            sim.step(steps)
+        f, g = sim.populations("f"), sim.populations("g")
+    o = oracle.Oracle(cfg, 32).init()
+    if steps:
+        o.step(steps)
+    J = (slice(None),) + I
+    err = np.abs(f[J] - o.populations("f")[J]).max() / pop_scale(cfg)
+    print(f"\n[mom populations] {case} steps={steps} f err {err:.2e}", flush=True)
+    assert err < 1e-5
+    assert np.abs(g[J] - o.populations("g")[J]).max() / 0.25 < 1e-5
+
+
+@pytest.mark.parametrize("zchunk", [1, 2, 3, 16, 64])
+def test_mom_zchunk_invariance(zchunk):
+    """Every cell sums the layers arriving from the planes below, its own and
+    above in that order whatever the z-chunk, so the chunk changes the
+    schedule, not one bit of the result."""
+    cfg = presets.round_jet(24, 24, 40, 6, D3Q27, storage=STORE_MOM, noise=NOISE_UNIFORM_Z)
+    ref = presets.round_jet(24, 24, 40, 6, D3Q27, storage=STORE_MOM, noise=NOISE_UNIFORM_Z)
+    cfg.zchunk = zchunk
+    ref.zchunk = 4
+    with pf.Simulation(cfg) as a, pf.Simulation(ref) as b:
+        a.init().step(10, save=True)
+        b.init().step(10, save=True)
+        for k in CORE:
+            np.testing.assert_array_equal(a.field(k), b.field(k))
+
+
+@pytest.mark.parametrize("case", ["droplet", "jet"])
+def test_mom_tile_width_invariance(case):
+    """The 64-wide and the 128-wide tile, with partial tiles in x and y (a
+    70 x 30 cross-section), give the same fields bit for bit."""
+    def mk():
+        if case == "droplet":
+            c = presets.static_droplet(n=30, radius=7.0, velocity_set=D3Q27, storage=STORE_MOM)
+            c.nx = 70
+            return c
+        return presets.round_jet(70, 30, 37, 8, D3Q27, storage=STORE_MOM, noise=NOISE_UNIFORM_Z)
+    a_cfg, b_cfg = mk(), mk()
+    a_cfg.tile_x, b_cfg.tile_x = 64, 128
+    with pf.Simulation(a_cfg) as a, pf.Simulation(b_cfg) as b:
+        a.init().step(12, save=True)
+        b.init().step(12, save=True)
+        for k in CORE:
+            np.testing.assert_array_equal(a.field(k), b.field(k))
+
+
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+def test_mom_phase_and_mass_conservation(Q):
+    """Droplet, 300 steps: total phi to 1e-6 relative (the phase sum is the
+    same kernel as in the population forms) and total mass to FP32
+    round-off."""
+    cfg = presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=STORE_MOM)
+    with pf.Simulation(cfg) as sim:
+        sim.init()
+        m0 = sim.phase_mass()
+        sim.step(1, save=True)
+        d0 = sim.diagnostics()
+        sim.step(299, save=True)
+        m1 = sim.phase_mass()
+        d1 = sim.diagnostics()
+    assert abs(m1 - m0) / m0 < 1e-6, (m0, m1)
+    assert abs(d1["mass"] - d0["mass"]) / d0["mass"] < 1e-6, (d0, d1)
+
+
+def test_mom_rejected_configs():
+    """The moment form takes the pair collision and the FD closure only."""
+    cfg = presets.round_jet(20, 20, 40, 6, D3Q27, storage=STORE_MOM)
+    cfg.kernel = pf.KERNEL_LITERAL
+    with pytest.raises(pf.PfError):
+        pf.Simulation(cfg)
+    cfg = presets.round_jet(20, 20, 40, 6, D3Q27, storage=STORE_MOM)
+    cfg.closure = pf.CLOSURE_MLP
+    with pytest.raises(pf.PfError):
+        pf.Simulation(cfg)

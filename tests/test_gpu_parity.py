@@ -588,13 +588,15 @@ def _graph_case(case):
         return presets.round_jet(24, 24, 40, 6, D3Q27, storage=STORE_FP16, noise=NOISE_UNIFORM_Z)
     if case == "jet_fp16_ne":
         return presets.round_jet(24, 24, 40, 6, D3Q27, storage=STORE_FP16_NE, noise=NOISE_UNIFORM_Z)
+    if case == "jet_mom":
+        return presets.round_jet(24, 24, 40, 6, D3Q27, storage=pf.STORE_MOM, noise=NOISE_UNIFORM_Z)
     cfg = presets.round_jet(24, 24, 40, 6, D3Q27, noise=NOISE_UNIFORM_Z)
     if case == "jet_closure":
         cfg.closure = pf.CLOSURE_MLP
     return cfg
 
 
-@pytest.mark.parametrize("case", ["droplet", "jet", "jet_fp16", "jet_fp16_ne", "jet_closure"])
+@pytest.mark.parametrize("case", ["droplet", "jet", "jet_fp16", "jet_fp16_ne", "jet_mom", "jet_closure"])
 def test_graph_replay_matches_direct_launches(case):
     """PF_STEP_GRAPH replays the phase sum, closure and step of each parity as
     a captured graph; the inflow kernel, which takes the time index, runs

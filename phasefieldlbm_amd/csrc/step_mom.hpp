@@ -4,7 +4,7 @@
 // with the populations kept as the coefficients of their senders' collision
 // output instead of stored one by one.  A workgroup marches a TX x TY column
 // of cells through its z-chunk.  Per plane:
-//   - the phi ring and the normals of k_step (same LDS layout);
+//   - a phi ring and the normals of plane w, as in k_step;
 //   - the coefficients of one sending plane w, with a 1-cell halo, are staged
 //     in LDS, and each thread rebuilds from them the populations its column
 //     receives from plane w: those with c_z = +1 belong to its cell in plane
@@ -27,20 +27,36 @@
 namespace pf {
 
 // Tiles of TX x 4 cells (TX * 4 threads): 64 x 4 by default, whose LDS
-// (56.8 KB) lets two workgroups share a CU.
+// (51.9 KB) and registers (at most 168) let three workgroups share a CU.
 constexpr int kMomTY = kMomRows;
+#ifndef PF_MOM_PREFETCH
+#define PF_MOM_PREFETCH 0
+#endif
+#ifndef PF_MOM_WAVES
+#define PF_MOM_WAVES 3
+#endif
+// PF_MOM_FAST: the hardware reciprocal and square root (1 ulp) for |grad
+// phi|, the normals, omega and 1/rho (recip / root of step_kernels.hpp).
+#ifndef PF_MOM_FAST
+#define PF_MOM_FAST 1
+#endif
 
-template <int Q, bool JET, bool SAVE, int TX>
-__global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) {
+template <int Q, bool JET, bool SAVE, bool LOCAL, int TX>
+__global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_step_mom(KParams p) {
     constexpr int NC = mom_nc<Q>();
+    constexpr bool kFast = PF_MOM_FAST != 0;
     constexpr int TY = kMomTY;
     constexpr int kThreads = TX * TY;
     constexpr int HX = TX + 4, HY = TY + 4;    // phi tile with 2-cell halo
     constexpr int NX1 = TX + 2, NY1 = TY + 2;  // normal and coefficient tiles with 1-cell halo
-    constexpr int kRing = 5;
+    // LDS: a 4-plane phi ring, 3 planes of n with a 1-cell halo, |grad phi|
+    // of 2 planes at the tile cells (it is read at the centre only), and the
+    // coefficients of one sending plane with a 1-cell halo
+    constexpr int kRing = 4;
     constexpr int kRec = NX1 * NY1;
     __shared__ float s_phi[kRing][HY * HX];
-    __shared__ float s_n[3][4][NY1 * NX1];
+    __shared__ float s_n[3][3][NY1 * NX1];
+    __shared__ float s_ind[2][kThreads];
     __shared__ float s_rec[NC][kRec];
 
     const int tid = threadIdx.x;
@@ -89,15 +105,16 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
         int cz = pw;
         if constexpr (!JET) cz = (pw < 1) ? nz - 2 : (pw > nz - 2 ? 1 : pw);
         cz = min(max(cz, 1), nz - 2);
+        const auto rs = rec_rsrc<Q>(p.rec_prev, plane, (uint32_t)cz);
 #pragma unroll
         for (int j = 0; j < kRIt; ++j) {
             const int i = min(tid + j * kThreads, kRec - 1);
             const int gx = x0 - 1 + i % NX1, gy = y0 - 1 + i / NX1;
             const int wx = gx <= 0 ? nx - 2 : (gx >= nx - 1 ? 1 : gx);
             const int wy = gy <= 0 ? ny - 2 : (gy >= ny - 1 ? 1 : gy);
-            const uint32_t c = (uint32_t)wx + (uint32_t)nx * ((uint32_t)wy + (uint32_t)ny * (uint32_t)cz);
+            const uint32_t xy = (uint32_t)wx + (uint32_t)nx * (uint32_t)wy;
 #pragma unroll
-            for (int k = 0; k < NC; ++k) v[j][k] = p.rec_prev[(size_t)k * p.N + c];
+            for (int k = 0; k < NC; ++k) v[j][k] = rec_ld(rs, xy, (uint32_t)k, plane);
         }
     };
     auto put_rec = [&](const float (&v)[kRIt][NC]) {
@@ -133,8 +150,8 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
                 const float gx_ = kAs2 * iso_sum<Q, 0>(val);
                 const float gy_ = kAs2 * iso_sum<Q, 1>(val);
                 const float gz_ = kAs2 * iso_sum<Q, 2>(val);
-                ind = sqrtf(gx_ * gx_ + gy_ * gy_ + gz_ * gz_);
-                const float inv = 1.0f / (ind + 1e-9f);
+                ind = root<kFast>(gx_ * gx_ + gy_ * gy_ + gz_ * gz_);
+                const float inv = recip<kFast>(ind + 1e-9f);
                 nrx = gx_ * inv;
                 nry = gy_ * inv;
                 nrz = gz_ * inv;
@@ -142,7 +159,7 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
             s_n[r][0][i] = nrx;
             s_n[r][1][i] = nry;
             s_n[r][2][i] = nrz;
-            s_n[r][3][i] = ind;
+            if (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY) s_ind[pz & 1][(ly - 1) * TX + (lx - 1)] = ind;
         }
     };
 
@@ -152,10 +169,16 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
     const int o1 = (ty + 1) * NX1 + (tx + 1);
     const uint32_t pcol = (uint32_t)min(x0 + tx, nx - 1) + (uint32_t)nx * (uint32_t)min(y0 + ty, ny - 1);
     auto bc_layer = [&](int base) {
+        const auto rb = __builtin_amdgcn_make_buffer_rsrc(p.bcf, (short)0, (int)(18u * plane * 4u), 0x00020000);
+        float v[9];
+        static_for<9>([&](auto oc) {
+            constexpr int o = decltype(oc)::value, cx = o % 3 - 1, cy = o / 3 - 1;
+            if constexpr (mom_has<Q>(cx, cy, 1)) v[o] = rec_ld(rb, pcol, (uint32_t)(base + o), plane);
+        });
         MomLayer g{0, 0, 0, 0, 0, 0, false};
         static_for<9>([&](auto oc) {
             constexpr int o = decltype(oc)::value, cx = o % 3 - 1, cy = o / 3 - 1;
-            if constexpr (mom_has<Q>(cx, cy, 1)) layer_add<cx, cy>(g, p.bcf[(uint32_t)(base + o) * plane + pcol]);
+            if constexpr (mom_has<Q>(cx, cy, 1)) layer_add<cx, cy>(g, v[o]);
         });
         return g;
     };
@@ -197,16 +220,16 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
 #pragma unroll
         for (int k = 0; k < 4; ++k) fetch_phi(zs - 2 + k, v[k]);
         float rv[kRIt][NC];
-        if (!local) fetch_rec(zs - 1, rv);
+        if constexpr (!LOCAL) fetch_rec(zs - 1, rv);
 #pragma unroll
         for (int k = 0; k < 4; ++k) put_phi(zs - 2 + k, v[k]);
-        if (!local) put_rec(rv);
+        if constexpr (!LOCAL) put_rec(rv);
     }
     lds_barrier();
     comp_normals(zs - 1);
     comp_normals(zs);
     MomAcc a_lo = mom_zero(), a_mid = mom_zero(), a_hi = mom_zero(), a_off = mom_zero();
-    if (!local) {
+    if constexpr (!LOCAL) {
         scatter(F{}, F{}, T{}, zs - 1, a_off, a_off, a_lo);
         lds_barrier();
         float rv[kRIt][NC];
@@ -220,24 +243,31 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
     const int x = x0 + tx, y = y0 + ty;
     const bool active = (x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2);
 
-    // The coefficients of sending plane z + 1 and phi of plane z + 2 are
-    // loaded one iteration ahead: their loads are in flight during the
-    // previous plane's work.
+    // PF_MOM_PREFETCH: the coefficients of sending plane z + 1 and phi of
+    // plane z + 2 are loaded during the previous plane's collision, so their
+    // loads are in flight while it computes (34 more registers there).
+    constexpr bool kAhead = PF_MOM_PREFETCH != 0;
     float rv[kRIt][NC];
     float phv[kPhiIt];
-    if (!local) fetch_rec(zs + 1, rv);
-    fetch_phi(zs + 2, phv);
+    if constexpr (kAhead) {
+        if constexpr (!LOCAL) fetch_rec(zs + 1, rv);
+        fetch_phi(zs + 2, phv);
+    }
     for (int z = zs; z < ze; ++z) {
+        if constexpr (!kAhead) {
+            if constexpr (!LOCAL) fetch_rec(z + 1, rv);
+            fetch_phi(z + 2, phv);
+        }
         put_phi(z + 2, phv);
-        if (!local) put_rec(rv);
+        if constexpr (!LOCAL) put_rec(rv);
         lds_barrier();
-        if (z + 1 < ze) {
-            if (!local) fetch_rec(z + 2, rv);
+        comp_normals(z + 1);
+        if constexpr (!LOCAL) scatter(T{}, T{}, T{}, z + 1, a_lo, a_mid, a_hi);
+        lds_barrier();
+        if (kAhead && z + 1 < ze) {
+            if constexpr (!LOCAL) fetch_rec(z + 2, rv);
             fetch_phi(z + 3, phv);
         }
-        comp_normals(z + 1);
-        if (!local) scatter(T{}, T{}, T{}, z + 1, a_lo, a_mid, a_hi);
-        lds_barrier();
 
         // the arrivals of plane z are complete
         MomAcc a = a_lo;
@@ -260,7 +290,7 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
         const float scz = iso_sum<Q, 2>([&](auto qc) { return nval(2, qc); });
         const float curvature = kAs2 * (scx + scy + scz);
         const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
-        const float ind = s_n[z % 3][3][o];
+        const float ind = s_ind[z & 1][tid];
         const float nrx = s_n[z % 3][0][o], nry = s_n[z % 3][1][o], nrz = s_n[z % 3][2][o];
         const float stCurv = -p.sigma * curvature * ind;
         const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
@@ -276,16 +306,18 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
             const float visc_local = (1.0f - phi) * p.visc_water + phi * p.visc_oil;
             const float tau_zmax = 0.5f + kAs2 * (visc_local * (1.0f + 100.0f));
             const float tau_eff = tau_phi + r * (tau_zmax - tau_phi);
-            omega = 1.0f / tau_eff;
+            omega = recip<kFast>(tau_eff);
         } else {
             omega = p.omega_ref;
         }
         const float omco = 1.0f - omega;
 
-        if (local) {
+        const uint32_t cxy = (uint32_t)x + (uint32_t)nx * (uint32_t)y;
+        if constexpr (LOCAL) {
+            const auto rs = rec_rsrc<Q>(p.rec_prev, plane, (uint32_t)z);
             float own[NC];
 #pragma unroll
-            for (int k = 0; k < NC; ++k) own[k] = p.rec_prev[(size_t)k * p.N + c];
+            for (int k = 0; k < NC; ++k) own[k] = rec_ld(rs, cxy, (uint32_t)k, plane);
             a = mom_local<Q>([&](int k) { return own[k]; });
         }
 
@@ -293,7 +325,7 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
         // force, deviatoric non-equilibrium stress
         const float m0 = a.m0;
         const float rho = m0 + 1.0f;
-        const float invRho = 1.0f / rho;
+        const float invRho = recip<kFast>(rho);
         const float ux = (a.jx + 0.5f * fsx) * invRho;
         const float uy = (a.jy + 0.5f * fsy) * invRho;
         const float uz = (a.jz + 0.5f * fsz) * invRho;
@@ -310,8 +342,11 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
         {
             float cf[NC];
             mom_coeffs<Q>(m0, rho, ux, uy, uz, fsx, fsy, fsz, pxx, pyy, pzz, pxy, pxz, pyz, omco, cf);
+            const auto rs = rec_rsrc<Q>(p.rec_cur, plane, (uint32_t)z);
 #pragma unroll
-            for (int k = 0; k < NC; ++k) __builtin_nontemporal_store(cf[k], p.rec_cur + (size_t)k * p.N + c);
+            for (int k = 0; k < NC; ++k)
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, cf[k]), rs, cxy * 4u,
+                                                      (uint32_t)k * plane * 4u, kStoreAux);
         }
 
         // ---- phase flux (src/lbm.cu:209-238, phase_flux.hpp)
@@ -342,23 +377,27 @@ __global__ __launch_bounds__(TX * kMomTY) void k_step_mom(KParams p, int local) 
     }
 }
 
-template <int Q, bool JET, bool SAVE>
-static hipError_t step_mom_t(const KParams& p, bool local, hipStream_t s) {
+template <int Q, bool JET, bool SAVE, bool LOCAL>
+static hipError_t step_mom_t(const KParams& p, hipStream_t s) {
     const uint32_t nblk = (uint32_t)p.ntx * (uint32_t)p.nty * (uint32_t)p.ntz;
     if (p.tile_x == 128)
-        hipLaunchKernelGGL((k_step_mom<Q, JET, SAVE, 128>), dim3(nblk), dim3(128 * kMomTY), 0, s, p, local ? 1 : 0);
+        hipLaunchKernelGGL((k_step_mom<Q, JET, SAVE, LOCAL, 128>), dim3(nblk), dim3(128 * kMomTY), 0, s, p);
     else if (p.tile_x == 64)
-        hipLaunchKernelGGL((k_step_mom<Q, JET, SAVE, 64>), dim3(nblk), dim3(64 * kMomTY), 0, s, p, local ? 1 : 0);
+        hipLaunchKernelGGL((k_step_mom<Q, JET, SAVE, LOCAL, 64>), dim3(nblk), dim3(64 * kMomTY), 0, s, p);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
+template <int Q, bool JET>
+static hipError_t step_mom_j(const KParams& p, bool local, hipStream_t s) {
+    if (p.save != 0) return local ? step_mom_t<Q, JET, true, true>(p, s) : step_mom_t<Q, JET, true, false>(p, s);
+    return local ? step_mom_t<Q, JET, false, true>(p, s) : step_mom_t<Q, JET, false, false>(p, s);
+}
+
 template <int Q>
 static hipError_t step_mom_q(const KParams& p, bool jet, bool local, hipStream_t s) {
-    const bool save = p.save != 0;
-    if (jet) return save ? step_mom_t<Q, true, true>(p, local, s) : step_mom_t<Q, true, false>(p, local, s);
-    return save ? step_mom_t<Q, false, true>(p, local, s) : step_mom_t<Q, false, false>(p, local, s);
+    return jet ? step_mom_j<Q, true>(p, local, s) : step_mom_j<Q, false>(p, local, s);
 }
 
 template <int Q>

@@ -118,7 +118,8 @@ def reduced_report():
     fix = np.load(FIXTURE)
     steps = [int(s) for s in fix["steps"]]
     rep = {}
-    for name, storage in (("fp16", pf.STORE_FP16), ("fp16_ne", pf.STORE_FP16_NE), ("fp32", pf.STORE_FP32)):
+    for name, storage in (("fp16", pf.STORE_FP16), ("fp16_ne", pf.STORE_FP16_NE), ("fp32", pf.STORE_FP32),
+                          ("mom", pf.STORE_MOM)):
         with pf.Simulation(_reduced_config(storage)) as sim:
             sim.init()
             done = 0
@@ -163,3 +164,16 @@ def test_fp16_ne_reduced_jet_against_fp64(reduced_report):
                 assert r["fp16_ne"][k] * 3 < r["fp16"][k], (s, k, r)
             else:
                 assert r["fp16_ne"][k] < min(2e-2, r["fp16"][k]), (s, k, r)
+
+
+@pytest.mark.gpu
+def test_mom_reduced_jet_against_fp64(reduced_report):
+    """The FP32 moment storage form on the reduced jet: within the FP32 bound
+    (1e-3 of FP64 on u and phi) at every snapshot up to 5000 steps, and the
+    phase mass within 1e-4 of FP64 up to 2000 steps, as FP32 population
+    storage."""
+    for s, r in reduced_report.items():
+        m = r["mom"]
+        assert m["rel_l2_u"] < 1e-3 and m["rel_l2_phi"] < 1e-3, (s, r)
+        if s <= 2000:
+            assert abs(m["phase_mass_drift"]) < 1e-4, (s, r)

@@ -7,6 +7,7 @@ src/flowCase/Jet.cuh:48-64, at the benchmark grids):
   D3Q27 256x256x1024, D = 32, FP32
   D3Q27 256x256x1024, D = 32, binary16 storage
   D3Q27 256x256x1024, D = 32, binary16 non-equilibrium storage
+  D3Q27 256x256x1024, D = 32, FP32 moment storage
 u = 0.05, Re = 5000, We = 500, uniform +-5 % u_z inflow noise, 200 steps
 (the jet front is ~10 cells from the nozzle).
 
@@ -30,14 +31,14 @@ Checked:
     of a stored population by one binary16 ulp, and the bar is 2e-3 u
     (measured 9.5e-4 after 200 steps).
   * z-chunk invariance at the chunk the automatic choice makes for each grid
-    (2 for the FP32 256x256x1024 jet, 16 for its binary16 form, 4 for
-    128x128x512) against 4 or 8-plane chunks, bit for bit.
+    (2 for the FP32 256x256x1024 jet, 16 for its binary16 and moment
+    forms, 4 for 128x128x512) against 4 or 8-plane chunks, bit for bit.
 """
 import numpy as np
 import pytest
 
 import phasefieldlbm_amd as pf
-from phasefieldlbm_amd import D3Q19, D3Q27, NOISE_NONE, STORE_FP16, STORE_FP16_NE, STORE_FP32, presets
+from phasefieldlbm_amd import D3Q19, D3Q27, NOISE_NONE, STORE_FP16, STORE_FP16_NE, STORE_FP32, STORE_MOM, presets
 
 pytestmark = pytest.mark.gpu
 
@@ -49,6 +50,7 @@ CASES = {
     "d3q27_256x256x1024_fp16": lambda **kw: presets.round_jet(256, 256, 1024, 32, D3Q27, storage=STORE_FP16, **kw),
     "d3q27_256x256x1024_fp16_ne": lambda **kw: presets.round_jet(256, 256, 1024, 32, D3Q27, storage=STORE_FP16_NE,
                                                                  **kw),
+    "d3q27_256x256x1024_mom": lambda **kw: presets.round_jet(256, 256, 1024, 32, D3Q27, storage=STORE_MOM, **kw),
 }
 
 
@@ -150,7 +152,8 @@ def test_fullsize_mirror_symmetry(case):
 
 
 @pytest.mark.parametrize("case,chunk", [("d3q27_256x256x1024", 4), ("d3q27_256x256x1024_fp16", 8),
-                                        ("d3q27_256x256x1024_fp16_ne", 8), ("d3q19_128x128x512", 8)])
+                                        ("d3q27_256x256x1024_fp16_ne", 8), ("d3q27_256x256x1024_mom", 8),
+                                        ("d3q19_128x128x512", 8)])
 def test_fullsize_zchunk_invariance(case, chunk):
     ref, alt = CASES[case](), CASES[case]()
     alt.zchunk = chunk

@@ -28,6 +28,17 @@ namespace pf {
 
 // Tiles of TX x 4 cells (TX * 4 threads): 64 x 4 by default, whose LDS
 // (51.9 KB) and registers (at most 168) let three workgroups share a CU.
+// Measured on MI355X (D3Q27 256x256x1024 jet, k_step per step):
+//   64 x 4, 3 workgroups per CU                      2.82 ms
+//   64 x 4, 2 per CU (no register cap)               2.85 ms (2.92 before
+//           the hardware reciprocal); 3.42 with a 5-plane ring and full
+//           normal planes (56.8 KB)
+//   128 x 4, 1 per CU (102 KB)                       3.75-3.95 ms
+//   32 x 4                                           3.09 ms
+//   PF_MOM_PREFETCH (next plane's loads issued a plane ahead: after the
+//   first barrier 3.6 ms with spills at 3 per CU, 3.13 ms at 2 per CU; after
+//   the scatter 2.89 ms)
+// z-chunks of 16, 32 and 64 planes are within 2 %.
 constexpr int kMomTY = kMomRows;
 #ifndef PF_MOM_PREFETCH
 #define PF_MOM_PREFETCH 0
@@ -36,10 +47,28 @@ constexpr int kMomTY = kMomRows;
 #define PF_MOM_WAVES 3
 #endif
 // PF_MOM_FAST: the hardware reciprocal and square root (1 ulp) for |grad
-// phi|, the normals, omega and 1/rho (recip / root of step_kernels.hpp).
+// phi|, the normals, omega and 1/rho (recip / root of step_kernels.hpp);
+// 2.85 -> 2.82 ms.
 #ifndef PF_MOM_FAST
 #define PF_MOM_FAST 1
 #endif
+
+// Workgroup b runs on XCD b % 8.  PF_MOM_BANDS: XCD k takes the k-th band
+// of nty / 8 tile rows and sweeps it x fastest, then row by row, then
+// z-chunk by z-chunk, so a tile's halo rows (coefficients and phi, 1.5x
+// its own cells on 4-row tiles) were fetched by the tiles beside it into
+// the same L2.  Grids whose tile rows do not divide into 8 bands keep the
+// hardware order.
+#ifndef PF_MOM_BANDS
+#define PF_MOM_BANDS 1
+#endif
+__device__ __forceinline__ uint32_t xcd_band_remap(uint32_t b, uint32_t ntx, uint32_t nty) {
+    if (PF_MOM_BANDS == 0 || nty % 8u != 0u) return b;
+    const uint32_t k = b & 7u, j = b >> 3;
+    const uint32_t rows = nty / 8u, layer = ntx * rows;
+    const uint32_t zc = j / layer, r = j - zc * layer;
+    return (zc * nty + k * rows + r / ntx) * ntx + r % ntx;
+}
 
 template <int Q, bool JET, bool SAVE, bool LOCAL, int TX>
 __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_step_mom(KParams p) {
@@ -62,7 +91,7 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
     const int tid = threadIdx.x;
     const int tx = tid & (TX - 1), ty = tid / TX;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x, (uint32_t)p.ntx);
+    const uint32_t lb = xcd_band_remap(blockIdx.x, (uint32_t)p.ntx, (uint32_t)p.nty);
     const int bx = (int)(lb % (uint32_t)p.ntx);
     const uint32_t rem = lb / (uint32_t)p.ntx;
     const int by = (int)(rem % (uint32_t)p.nty);

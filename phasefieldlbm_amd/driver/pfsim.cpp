@@ -46,6 +46,7 @@ void usage(const char* argv0) {
               << "  --grid NX NY NZ, --diam D, --radius R, --sigma S\n"
               << "  --fp16         binary16 population storage\n"
               << "  --fp16-ne      binary16 storage of the non-equilibrium remainder\n"
+              << "  --moments      FP32 storage of each cell's collision-output coefficients\n"
               << "  --noise gaussian|uniform|none, --noise-amp A, --seed N\n"
               << "  --closure FILE learned curvature closure weights (6017 little-endian float32)\n"
               << "  --derived LIST average,reynolds,vorticity: derived fields, appended to the output\n"
@@ -168,7 +169,7 @@ void print_metadata(const Run& r, const pf_config& c, float sigma) {
               << "Mach:                  " << ma << '\n'
               << "Dispersed phase tau:   " << tau_d << '\n'
               << "Continuous phase tau:  " << tau_c << '\n'
-              << "Storage:               " << (c.storage == PF_STORE_FP16 ? "FP16" : (c.storage == PF_STORE_FP16_NE ? "FP16 (non-equilibrium)" : "FP32")) << '\n'
+              << "Storage:               " << (c.storage == PF_STORE_FP16 ? "FP16" : (c.storage == PF_STORE_FP16_NE ? "FP16 (non-equilibrium)" : (c.storage == PF_STORE_MOM ? "FP32 (moments)" : "FP32"))) << '\n'
               << "Curvature closure:     " << (c.closure == PF_CLOSURE_MLP ? "learned" : "finite difference")
               << '\n'
               << "-----------------------------------------------------------------------------\n\n";
@@ -231,6 +232,7 @@ int main(int argc, char* argv[]) {
         else if (a == "--sigma") { need(1); sigma_override = (float)std::atof(argv[++i]); }
         else if (a == "--fp16") cfg.storage = PF_STORE_FP16;
         else if (a == "--fp16-ne") cfg.storage = PF_STORE_FP16_NE;
+        else if (a == "--moments") cfg.storage = PF_STORE_MOM;
         else if (a == "--noise") {
             need(1);
             const std::string n = argv[++i];

@@ -259,7 +259,7 @@ def main():
         out["laplace"] = [laplace(pf, presets, R) for R in (8, 12, 16)]
         # the other configurations of the workload set, FP32, for reference
         others = {}
-        for name in ("jet_d3q19_128x128x512", "droplet_d3q19_64"):
+        for name in ("jet_d3q19_128x128x512", "jet_d3q19_128x128x512_mom", "droplet_d3q19_64"):
             make, _ = presets.WORKLOADS[name]
             c = make()
             w, _, _ = run_form(pf, c, args.steps, args.warmup)

@@ -61,6 +61,7 @@ def jet_d3q27_128x128x512(storage: int = STORE_FP32) -> Config:
 WORKLOADS = {
     "droplet_d3q19_64": (lambda: static_droplet(), 2000),
     "jet_d3q19_128x128x512": (jet_d3q19_128x128x512, 10000),
+    "jet_d3q19_128x128x512_mom": (lambda: round_jet(128, 128, 512, 16, D3Q19, storage=STORE_MOM), 10000),
     "jet_d3q27_256x256x1024": (jet_d3q27_256x256x1024, 20000),
     "jet_d3q27_256x256x1024_fp16": (lambda: jet_d3q27_256x256x1024(STORE_FP16), 20000),
     "jet_d3q27_128x128x512_fp16": (lambda: jet_d3q27_128x128x512(STORE_FP16), 5000),

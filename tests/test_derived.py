@@ -50,10 +50,12 @@ def test_vorticity_reference_formula():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["jet", "droplet"])
+@pytest.mark.parametrize("case", ["jet", "droplet", "jet_moments"])
 def test_running_means_and_vorticity(case):
     if case == "jet":
         cfg = presets.round_jet(24, 24, 48, 6, pf.D3Q19)
+    elif case == "jet_moments":  # the moment storage form materialises every field each step too
+        cfg = presets.round_jet(24, 24, 48, 6, pf.D3Q27, storage=pf.STORE_MOM)
     else:
         cfg = presets.static_droplet(n=20, radius=5.0)
     steps = 12

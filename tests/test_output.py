@@ -195,6 +195,26 @@ def test_driver_fp16_jet(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_moments_jet(tmp_path):
+    """pfsim JET D3Q27 <ID> --moments: the FP32 moment storage form through
+    the driver; the metadata names it and the final fields equal the same
+    steps run through the Python binding in that form."""
+    import phasefieldlbm_amd as pf
+    r = _run(["JET", "D3Q27", "005", "--preset", "jet_d3q27_128x128x512", "--moments", "--steps", "30", "--save",
+              "30", "--fields", "uz,phi", "--format", "bin"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "moments" in r.stdout
+    d = tmp_path / "bin" / "JET" / "D3Q27" / "005"
+    cfg = pf.presets.jet_d3q27_128x128x512(pf.STORE_MOM)
+    with pf.Simulation(cfg) as sim:
+        sim.init().step(31, save=True, t0=0)
+        for k in ("uz", "phi"):
+            got = vtk.read_bin(str(d / f"{k}000030.bin"), cfg.shape)
+            assert np.isfinite(got).all()
+            np.testing.assert_array_equal(got, sim.field(k))
+
+
+@pytest.mark.gpu
 def test_driver_learned_closure(tmp_path):
     from phasefieldlbm_amd import closure
     r = _run(["DROPLET", "D3Q19", "002", "--preset", "droplet_d3q19_64", "--steps", "30", "--save", "30",

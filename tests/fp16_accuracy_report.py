@@ -1,7 +1,7 @@
-"""Errors of the three storage forms against an FP64 fixture of a jet.
+"""Errors of the storage forms against an FP64 fixture of a jet.
 
     python tests/fp16_accuracy_report.py --fixture F.npz \
-        --workload jet_d3q27_128x128x512_fp16 [--forms fp32,fp16,fp16_ne]
+        --workload jet_d3q27_128x128x512_fp16 [--forms fp32,fp16,fp16_ne,mom]
 
 F.npz holds the FP64 oracle's interior u_x, u_y, u_z and phi at the steps
 its "steps" array lists, with the FP64 sums of phi (the format of
@@ -33,7 +33,7 @@ import phasefieldlbm_amd as pf  # noqa: E402
 from phasefieldlbm_amd import presets  # noqa: E402
 from test_fp16_accuracy import _reduced_metrics  # noqa: E402
 
-FORMS = {"fp32": pf.STORE_FP32, "fp16": pf.STORE_FP16, "fp16_ne": pf.STORE_FP16_NE}
+FORMS = {"fp32": pf.STORE_FP32, "fp16": pf.STORE_FP16, "fp16_ne": pf.STORE_FP16_NE, "mom": pf.STORE_MOM}
 
 
 def run(fixture: str, workload: str, forms) -> dict:
@@ -63,7 +63,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--fixture", required=True)
     ap.add_argument("--workload", required=True)
-    ap.add_argument("--forms", default="fp32,fp16,fp16_ne")
+    ap.add_argument("--forms", default="fp32,fp16,fp16_ne,mom")
     args = ap.parse_args()
     forms = [f for f in args.forms.split(",") if f]
     rep = run(args.fixture, args.workload, forms)

@@ -6,11 +6,11 @@ Workload: D3Q27 round jet on 40x40x128 cells (ghost shell included), nozzle
 D = 8, u = 0.05, Re = 5000, We = 500, uniform +-5 % u_z inflow noise with the
 default seed: the 128x128x512 / D = 16 accuracy case of the binary16 form at
 a quarter of the resolution, small enough for the FP64 oracle to run in a few
-minutes.  Saved: u_x, u_y, u_z and phi of the interior cells after 500,
-1000, 2000 and 5000 steps as float32 rounded to 16 mantissa bits (relative
-rounding <= 2^-17 = 7.6e-6 per value, which keeps the file at a few MB; the
-errors the fixture measures are 1e-5 and above), and the FP64 sums of phi
-and rho.  Test infrastructure: the accuracy reference of
+minutes.  Saved: u_x, u_y, u_z, phi and the density deviation rho - 1 of
+the interior cells after 500, 1000, 2000 and 5000 steps as float32 rounded
+to 16 mantissa bits (relative rounding <= 2^-17 = 7.6e-6 per value, which
+keeps the file at a few MB; the errors the fixture measures are 1e-5 and
+above), and the FP64 sums of phi and rho.  Test infrastructure: the accuracy reference of
 tests/test_fp16_accuracy.py.
 """
 from __future__ import annotations
@@ -66,6 +66,8 @@ def main():
             if k == "phi":
                 out[f"sum_phi_{s}"] = np.float64(a.sum())
             out[f"{k}_{s}"] = round_mantissa(a.astype(np.float32), MANTISSA_BITS)
+        # rho - 1 in FP64 before the rounding: rho itself is 1 to ~1e-3
+        out[f"drho_{s}"] = round_mantissa((o.field("rho")[I] - 1.0).astype(np.float32), MANTISSA_BITS)
         print(f"step {s}: {time.time() - t0:.0f} s", flush=True)
     np.savez_compressed(OUT, **out)
     print(OUT, os.path.getsize(OUT), "bytes")

@@ -11,6 +11,8 @@ interior cells is
 
     rel_l2(u)   = ||u - u64||_2 / ||u64||_2       (vector field)
     rel_l2(phi) = ||phi - phi64||_2 / ||phi64||_2
+    rel_l2(rho) = ||rho - rho64||_2 / ||rho64||_2, and the same of the
+                  deviation rho - 1 (rho itself is 1 to ~1e-3)
     rel_linf    = max |a - a64| / max |a64|
     phase-mass drift (sum phi - sum phi64) / sum phi64
 
@@ -51,14 +53,19 @@ def test_metrics_definition():
     got["phi"] = rng.uniform(0.1, 1.0, size=shape).astype(np.float32)
     fix = {f"{k}_7": v[I].copy() for k, v in got.items()}
     fix["sum_phi_7"] = np.float64(got["phi"][I].astype(np.float64).sum())
+    drho = (1e-3 * rng.normal(size=shape)).astype(np.float64)
+    got["rho"] = 1.0 + drho
+    fix["drho_7"] = drho[I].copy()
     m0 = _reduced_metrics(got, fix, 7)
     assert all(abs(v) < 1e-12 for v in m0.values()), m0
     got["phi"] = got["phi"] * np.float32(1.001)
     got["ux"][0] += 5.0  # ghost shell
+    got["rho"] = 1.0 + 1.001 * drho
     m = _reduced_metrics(got, fix, 7)
-    for k in ("rel_l2_phi", "rel_linf_phi", "phase_mass_drift"):
+    for k in ("rel_l2_phi", "rel_linf_phi", "phase_mass_drift", "rel_l2_drho"):
         assert abs(m[k] - 1e-3) < 1e-6, (k, m)
     assert m["rel_l2_u"] == 0.0
+    assert m["rel_l2_rho"] < 2e-6  # rho itself is 1 to ~1e-3
 
 
 def _check(r, s):
@@ -83,13 +90,19 @@ def _reduced_metrics(got, fix, s):
     u = np.stack([got[k][I] for k in ("ux", "uy", "uz")]).astype(np.float64)
     u64 = np.stack([fix[f"{k}_{s}"] for k in ("ux", "uy", "uz")]).astype(np.float64)
     ph, ph64 = got["phi"][I].astype(np.float64), fix[f"phi_{s}"].astype(np.float64)
-    return {
+    m = {
         "rel_l2_u": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
         "rel_linf_u": float(np.abs(u - u64).max() / np.abs(u64).max()),
         "rel_l2_phi": float(np.linalg.norm(ph - ph64) / np.linalg.norm(ph64)),
         "rel_linf_phi": float(np.abs(ph - ph64).max() / np.abs(ph64).max()),
         "phase_mass_drift": float((ph.sum() - float(fix[f"sum_phi_{s}"])) / float(fix[f"sum_phi_{s}"])),
     }
+    if "rho" in got and f"drho_{s}" in fix:
+        d, d64 = got["rho"][I].astype(np.float64) - 1.0, fix[f"drho_{s}"].astype(np.float64)
+        m["rel_l2_rho"] = float(np.linalg.norm(d - d64) / np.linalg.norm(d64 + 1.0))
+        m["rel_linf_rho"] = float(np.abs(d - d64).max() / np.abs(d64 + 1.0).max())
+        m["rel_l2_drho"] = float(np.linalg.norm(d - d64) / np.linalg.norm(d64))
+    return m
 
 
 @pytest.mark.slow
@@ -126,7 +139,7 @@ def reduced_report():
             for s in steps:
                 sim.step(s - done, save=True)
                 done = s
-                got = {k: sim.field(k) for k in ("ux", "uy", "uz", "phi")}
+                got = {k: sim.field(k) for k in ("ux", "uy", "uz", "phi", "rho")}
                 rep.setdefault(s, {})[name] = _reduced_metrics(got, fix, s)
     print(json.dumps({"fp16_accuracy_vs_fp64_40x40x128": rep}), flush=True)
     return rep

@@ -271,7 +271,10 @@ int pf_create(const pf_config* cfg, pf_ctx** out) {
     if (g.storage != PF_STORE_FP32 && g.storage != PF_STORE_FP16 && g.storage != PF_STORE_FP16_NE &&
         g.storage != PF_STORE_MOM)
         return PF_ERR_CONFIG;
-    if (g.storage == PF_STORE_MOM && (g.kernel == PF_KERNEL_LITERAL || g.closure == PF_CLOSURE_MLP))
+    // the moment form: pair collision and FD curvature, and one plane of
+    // coefficients addressable by a 32-bit buffer range (step_mom.hpp)
+    if (g.storage == PF_STORE_MOM && (g.kernel == PF_KERNEL_LITERAL || g.closure == PF_CLOSURE_MLP ||
+                                      (uint64_t)g.nx * (uint64_t)g.ny * 17u * 4u >= ((uint64_t)1 << 32)))
         return PF_ERR_CONFIG;
     if (g.storage == PF_STORE_FP16_NE) {
         // pair collision only; the nozzle must lie inside the lateral ghost

@@ -72,6 +72,23 @@ def test_ne_storage_constraints():
     assert e.value.status == -1
 
 
+def test_moment_storage_constraints():
+    """The moment storage form takes the pair collision and the FD curvature
+    only, and planes whose coefficients fit a 32-bit buffer range
+    (17 x 4 B x nx*ny < 2^32); checked before any device is touched."""
+    for field, value in (("kernel", pf.KERNEL_LITERAL), ("closure", pf.CLOSURE_MLP)):
+        cfg = pf.presets.round_jet(24, 24, 40, 6, pf.D3Q27, storage=pf.STORE_MOM)
+        setattr(cfg, field, value)
+        with pytest.raises(pf.PfError) as e:
+            pf.Simulation(cfg)
+        assert e.value.status == -1, field
+    cfg = pf.presets.static_droplet(n=8, radius=2.0, velocity_set=pf.D3Q27, storage=pf.STORE_MOM)
+    cfg.nx, cfg.ny, cfg.nz = 8000, 8000, 4
+    with pytest.raises(pf.PfError) as e:
+        pf.Simulation(cfg)
+    assert e.value.status == -1
+
+
 def test_status_strings():
     lib = _abi.load()
     for code in range(0, -8, -1):

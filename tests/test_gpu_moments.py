@@ -130,15 +130,3 @@ def test_mom_phase_and_mass_conservation(Q):
         d1 = sim.diagnostics()
     assert abs(m1 - m0) / m0 < 1e-6, (m0, m1)
     assert abs(d1["mass"] - d0["mass"]) / d0["mass"] < 1e-6, (d0, d1)
-
-
-def test_mom_rejected_configs():
-    """The moment form takes the pair collision and the FD closure only."""
-    cfg = presets.round_jet(20, 20, 40, 6, D3Q27, storage=STORE_MOM)
-    cfg.kernel = pf.KERNEL_LITERAL
-    with pytest.raises(pf.PfError):
-        pf.Simulation(cfg)
-    cfg = presets.round_jet(20, 20, 40, 6, D3Q27, storage=STORE_MOM)
-    cfg.closure = pf.CLOSURE_MLP
-    with pytest.raises(pf.PfError):
-        pf.Simulation(cfg)

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_init_mom(KParams p) {
     float cf[mom_nc<Q>()];
     mom_coeffs<Q>(rho - 1.0f, rho, ux, uy, uz, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, cf);
     for (int k = 0; k < mom_nc<Q>(); ++k) {
-        const uint64_t e = rec_elem<Q>((uint32_t)k, i, (uint32_t)p.nx * (uint32_t)p.ny);
+        const uint64_t e = rec_elem<Q>((uint32_t)k, i, (uint32_t)p.nx, (uint32_t)p.ny);
         p.rec_cur[e] = cf[k];
         const_cast<float*>(p.rec_prev)[e] = cf[k];
     }
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(256) void k_gather_mom(KParams p, float* out) {
             } else if (JET && !LOCAL && kCZ[q] == -1 && z == nz - 2) {
                 v = p.bcf[(uint32_t)(9 + (kCY[q] + 1) * 3 + kCX[q] + 1) * plane + pi];
             } else {
-                v = mom_pop<Q, q>([&](int k) { return p.rec_prev[rec_elem<Q>((uint32_t)k, w, plane)]; });
+                v = mom_pop<Q, q>([&](int k) { return p.rec_prev[rec_elem<Q>((uint32_t)k, w, (uint32_t)nx, (uint32_t)ny)]; });
             }
         }
         out[(size_t)q * p.N + i] = v;

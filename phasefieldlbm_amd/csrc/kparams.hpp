@@ -126,7 +126,7 @@ struct KParams {
     const uint2* mom_prev;
     uint2* mom_cur;
     // moment form (mom != 0, mom.hpp): the FP32 coefficients of every
-    // cell's collision output, interleaved per z-plane (rec_elem); rec_prev holds those the
+    // cell's collision output, interleaved per (y, z) row (rec_elem); rec_prev holds those the
     // arriving populations are rebuilt from, rec_cur receives this step's
     // (the host swaps them).  bcf: the jet's boundary populations for the
     // next step, 18 x nx*ny: the 9 c_z = +1 directions entering z = 1, then

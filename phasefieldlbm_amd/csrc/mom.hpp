@@ -44,23 +44,28 @@ enum : int {
 template <int Q>
 __host__ __device__ constexpr int mom_nc() { return Q == 27 ? 17 : 10; }
 
-// Coefficient arrays are interleaved per z-plane: coefficient k of the cell
-// with plane index z and in-plane offset xy = x + nx*y sits at
-// (z*NC + k)*plane + xy, plane = nx*ny.  A plane's NC coefficients form one
-// block, addressed from a per-plane buffer resource with the slot as a
-// scalar offset and xy as a 32-bit vector offset.
+// Coefficient arrays are interleaved per (y, z) row, as the population
+// arrays: coefficient k of cell (x, y, z) sits at
+// z*NC*nx*ny + y*NC*nx + k*nx + x, so the NC coefficients of a row of cells
+// are NC contiguous runs.  Accessed from a per-plane buffer resource with
+// y*NC*nx + x as the 32-bit vector offset and k*nx as the scalar offset.
 template <int Q>
-__device__ __forceinline__ uint64_t rec_elem(uint32_t k, uint32_t c, uint32_t plane) {
-    const uint32_t z = c / plane, xy = c - z * plane;
-    return ((uint64_t)z * mom_nc<Q>() + k) * plane + xy;
+__device__ __forceinline__ uint64_t rec_elem(uint32_t k, uint32_t c, uint32_t nx, uint32_t ny) {
+    const uint32_t plane = nx * ny, z = c / plane, rem = c - z * plane, y = rem / nx, x = rem - y * nx;
+    return ((uint64_t)z * ny + y) * ((uint64_t)mom_nc<Q>() * nx) + (uint64_t)k * nx + x;
+}
+template <int Q>
+__device__ __forceinline__ uint32_t rec_off(uint32_t x, uint32_t y, uint32_t nx) {
+    return y * (uint32_t)mom_nc<Q>() * nx + x;
 }
 template <int Q>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rec_rsrc(const float* base, uint32_t plane, uint32_t z) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base) + (uint64_t)z * (uint32_t)mom_nc<Q>() * plane,
                                              (short)0, (int)((uint32_t)mom_nc<Q>() * plane * 4u), 0x00020000);
 }
-__device__ __forceinline__ float rec_ld(__amdgpu_buffer_rsrc_t r, uint32_t xy, uint32_t k, uint32_t plane) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, xy * 4u, k * plane * 4u, 0));
+// Element `off` of run `slot` (runs `stride` elements apart) of a buffer.
+__device__ __forceinline__ float slot_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t slot, uint32_t stride) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off * 4u, slot * stride * 4u, 0));
 }
 
 // Raw moments of the populations arriving at a cell: sum f~, sum f~ c,

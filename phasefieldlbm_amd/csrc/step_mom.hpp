@@ -150,9 +150,9 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
             const int gx = x0 - 1 + i % NX1, gy = y0 - 1 + i / NX1;
             const int wx = gx <= 0 ? nx - 2 : (gx >= nx - 1 ? 1 : gx);
             const int wy = gy <= 0 ? ny - 2 : (gy >= ny - 1 ? 1 : gy);
-            const uint32_t xy = (uint32_t)wx + (uint32_t)nx * (uint32_t)wy;
+            const uint32_t ro = rec_off<Q>((uint32_t)wx, (uint32_t)wy, (uint32_t)nx);
 #pragma unroll
-            for (int k = 0; k < NC; ++k) v[j][k] = rec_ld(rs, xy, (uint32_t)k, plane);
+            for (int k = 0; k < NC; ++k) v[j][k] = slot_ld(rs, ro, (uint32_t)k, (uint32_t)nx);
         }
     };
     auto put_rec = [&](const float (&v)[kRIt][NC]) {
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
         float v[9];
         static_for<9>([&](auto oc) {
             constexpr int o = decltype(oc)::value, cx = o % 3 - 1, cy = o / 3 - 1;
-            if constexpr (mom_has<Q>(cx, cy, 1)) v[o] = rec_ld(rb, pcol, (uint32_t)(base + o), plane);
+            if constexpr (mom_has<Q>(cx, cy, 1)) v[o] = slot_ld(rb, pcol, (uint32_t)(base + o), plane);
         });
         MomLayer g{0, 0, 0, 0, 0, 0, false};
         static_for<9>([&](auto oc) {
@@ -350,12 +350,12 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
         }
         const float omco = 1.0f - omega;
 
-        const uint32_t cxy = (uint32_t)x + (uint32_t)nx * (uint32_t)y;
         if constexpr (LOCAL) {
             const auto rs = rec_rsrc<Q>(p.rec_prev, plane, (uint32_t)z);
             float own[NC];
 #pragma unroll
-            for (int k = 0; k < NC; ++k) own[k] = rec_ld(rs, cxy, (uint32_t)k, plane);
+            for (int k = 0; k < NC; ++k)
+                own[k] = slot_ld(rs, rec_off<Q>((uint32_t)x, (uint32_t)y, (uint32_t)nx), (uint32_t)k, (uint32_t)nx);
             a = mom_local<Q>([&](int k) { return own[k]; });
         }
 
@@ -383,8 +383,9 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
             const auto rs = rec_rsrc<Q>(p.rec_cur, plane, (uint32_t)z);
 #pragma unroll
             for (int k = 0; k < NC; ++k)
-                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, cf[k]), rs, cxy * 4u,
-                                                      (uint32_t)k * plane * 4u, kStoreAux);
+                __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, cf[k]), rs,
+                                                      rec_off<Q>((uint32_t)x, (uint32_t)y, (uint32_t)nx) * 4u,
+                                                      (uint32_t)k * (uint32_t)nx * 4u, kStoreAux);
         }
 
         // ---- phase flux (src/lbm.cu:209-238, phase_flux.hpp)

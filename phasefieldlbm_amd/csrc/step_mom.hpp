@@ -62,16 +62,20 @@ constexpr int kMomTY = kMomRows;
 #define PF_MOM_FAST 1
 #endif
 
-// Workgroup b runs on XCD b % 8.  PF_MOM_BANDS: XCD k takes the k-th band
+// Workgroup b runs on XCD b % 8.  PF_MOM_BANDS = 1: XCD k takes the k-th band
 // of nty / 8 tile rows and sweeps it x fastest, then row by row, then
 // z-chunk by z-chunk, so a tile's halo rows (coefficients and phi, 1.5x
 // its own cells on 4-row tiles) were fetched by the tiles beside it into
 // the same L2.  Grids whose tile rows do not divide into 8 bands keep the
-// hardware order.
+// hardware order.  PF_MOM_BANDS = 2 deals whole tile rows round-robin as
+// k_step does: the same time with the row-interleaved coefficients (2.52 ms
+// at 16-plane chunks), the hardware order 2.98 ms with the earlier
+// per-plane layout.
 #ifndef PF_MOM_BANDS
 #define PF_MOM_BANDS 1
 #endif
-__device__ __forceinline__ uint32_t xcd_band_remap(uint32_t b, uint32_t ntx, uint32_t nty) {
+__device__ __forceinline__ uint32_t xcd_band_remap(uint32_t b, uint32_t nblk, uint32_t ntx, uint32_t nty) {
+    if (PF_MOM_BANDS == 2) return xcd_remap(b, nblk, ntx);  // whole tile rows dealt round-robin (k_step)
     if (PF_MOM_BANDS == 0 || nty % 8u != 0u) return b;
     const uint32_t k = b & 7u, j = b >> 3;
     const uint32_t rows = nty / 8u, layer = ntx * rows;
@@ -100,7 +104,7 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
     const int tid = threadIdx.x;
     const int tx = tid & (TX - 1), ty = tid / TX;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
-    const uint32_t lb = xcd_band_remap(blockIdx.x, (uint32_t)p.ntx, (uint32_t)p.nty);
+    const uint32_t lb = xcd_band_remap(blockIdx.x, gridDim.x, (uint32_t)p.ntx, (uint32_t)p.nty);
     const int bx = (int)(lb % (uint32_t)p.ntx);
     const uint32_t rem = lb / (uint32_t)p.ntx;
     const int by = (int)(rem % (uint32_t)p.nty);

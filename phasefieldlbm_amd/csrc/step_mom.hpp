@@ -29,6 +29,9 @@ namespace pf {
 // Tiles of TX x 4 cells (TX * 4 threads): 64 x 4 by default, whose LDS
 // (51.9 KB) and registers (at most 168) let three workgroups share a CU.
 // Measured on MI355X (D3Q27 256x256x1024 jet, k_step per step):
+//   64 x 4, 3 workgroups per CU, coefficients interleaved per row, 32-plane
+//   z-chunks                                         2.49 ms
+// and, with the earlier layout of one plane-sized array per coefficient:
 //   64 x 4, 3 workgroups per CU                      2.82 ms
 //   64 x 4, 2 per CU (no register cap)               2.85 ms (2.92 before
 //           the hardware reciprocal); 3.42 with a 5-plane ring and full
@@ -44,7 +47,8 @@ namespace pf {
 //   a scheduling fence after each sender in the scatter (80 instead of
 //   168 registers): 3.07-3.13 ms
 //   the ghost shell holding copies of the cells it images, so that the
-//   coefficients are staged with 16-byte loads and no wrap: 2.93 ms
+//   coefficients are staged with 16-byte loads and no wrap: 2.93 ms (2.67
+//   ms against 2.49 with the row-interleaved coefficients)
 //   (timing copy with the coefficient staging left out: 1.93 ms; with the
 //   scatter left out 2.73, with the normals and curvature left out 2.68)
 // z-chunks of 16, 32 and 64 planes are within 2 %.

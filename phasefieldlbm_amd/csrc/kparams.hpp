@@ -70,7 +70,10 @@ constexpr int kTXNarrow = 64;
 static_assert(kThreads % kTXNarrow == 0, "narrow tile");
 constexpr int tile_width(int nx) { return nx <= kTXNarrow ? kTXNarrow : kTX; }
 // The moment form's step tiles are tile_x x kMomRows cells (step_mom.hpp).
-constexpr int kMomRows = 4;
+#ifndef PF_MOM_ROWS
+#define PF_MOM_ROWS 4
+#endif
+constexpr int kMomRows = PF_MOM_ROWS;
 
 // Scale K of the binary16 non-equilibrium form (ne.hpp).
 constexpr float kNeK = 4096.0f;

@@ -31,6 +31,8 @@ namespace pf {
 // Measured on MI355X (D3Q27 256x256x1024 jet, k_step per step):
 //   64 x 4, 3 workgroups per CU, coefficients interleaved per row, 32-plane
 //   z-chunks                                         2.49 ms
+//   64 x 6 (PF_MOM_ROWS=6, 68.9 KB, 2 per CU), 43 tile rows dealt
+//   round-robin or in hardware order                 4.45-4.62 ms
 // and, with the earlier layout of one plane-sized array per coefficient:
 //   64 x 4, 3 workgroups per CU                      2.82 ms
 //   64 x 4, 2 per CU (no register cap)               2.85 ms (2.92 before

@@ -33,6 +33,10 @@ namespace pf {
 //   z-chunks                                         2.49 ms
 //   64 x 6 (PF_MOM_ROWS=6, 68.9 KB, 2 per CU), 43 tile rows dealt
 //   round-robin or in hardware order                 4.45-4.62 ms
+//   the next sending plane's coefficients loaded straight into LDS (buffer
+//   loads with the lds modifier) right after the scatter, and phi of the
+//   plane after it into registers, both in flight during the collision:
+//   2.47 ms, no gain, so the step does not wait on its loads' latency
 // and, with the earlier layout of one plane-sized array per coefficient:
 //   64 x 4, 3 workgroups per CU                      2.82 ms
 //   64 x 4, 2 per CU (no register cap)               2.85 ms (2.92 before

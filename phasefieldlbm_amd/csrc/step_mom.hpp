@@ -33,6 +33,8 @@ namespace pf {
 //   z-chunks                                         2.49 ms
 //   64 x 6 (PF_MOM_ROWS=6, 68.9 KB, 2 per CU), 43 tile rows dealt
 //   round-robin or in hardware order                 4.45-4.62 ms
+//   32 x 8 (44.3 KB, 125 VGPRs, 3 per CU; a 34 x 10 halo tile, 1.33x its
+//   cells against 1.55x)                             2.56 ms
 //   the next sending plane's coefficients loaded straight into LDS (buffer
 //   loads with the lds modifier) right after the scatter, and phi of the
 //   plane after it into registers, both in flight during the collision:

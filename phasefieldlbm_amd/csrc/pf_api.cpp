@@ -185,10 +185,10 @@ int choose_zchunk(const pf_config& g) {
     const long cols = (long)((g.nx + tw - 1) / tw) * ((g.ny + th - 1) / th);
     // the binary16 and moment forms march long chunks (the moment form
     // stages two sending planes more than it updates per chunk; on the
-    // 256x256x1024 jet its step took 2.49 ms at 32 planes, 2.52 at 16 and
-    // 2.72 at 8)
+    // 256x256x1024 jet its step took 2.37 ms at 16 planes and 2.40-2.42 at
+    // 32, 2.72 at 8 with an earlier kernel)
     const bool fp16 = g.storage == PF_STORE_FP16 || g.storage == PF_STORE_MOM;
-    const long target = fp16 ? 8192 : 32768;
+    const long target = g.storage == PF_STORE_MOM ? 16384 : (fp16 ? 8192 : 32768);
     long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
     if (zc < 4) zc = 4;
     if (zc > 64) zc = 64;

@@ -27,10 +27,17 @@
 namespace pf {
 
 // Tiles of TX x 4 cells (TX * 4 threads): 64 x 4 by default, whose LDS
-// (51.9 KB) and registers (at most 168) let three workgroups share a CU.
+// (40.3 KB) and registers (at most 128) let four workgroups share a CU.
 // Measured on MI355X (D3Q27 256x256x1024 jet, k_step per step):
-//   64 x 4, 3 workgroups per CU, coefficients interleaved per row, 32-plane
-//   z-chunks                                         2.49 ms
+//   64 x 4, 4 workgroups per CU, the curvature marched in z (one plane of
+//   normals in LDS) and a 3-plane phi ring, 16-plane z-chunks
+//                                                    2.37 ms
+//   the same with 32-plane z-chunks 2.40-2.42 ms, at 3 workgroups per CU
+//   (134 registers) 2.42-2.45 ms
+// and with the curvature summed over three planes of normals in LDS and a
+// 4-plane phi ring (51.9 KB, 3 workgroups per CU):
+//   64 x 4, coefficients interleaved per row, 32-plane z-chunks
+//                                                    2.49 ms
 //   64 x 6 (PF_MOM_ROWS=6, 68.9 KB, 2 per CU), 43 tile rows dealt
 //   round-robin or in hardware order                 4.45-4.62 ms
 //   32 x 8 (44.3 KB, 125 VGPRs, 3 per CU; a 34 x 10 halo tile, 1.33x its
@@ -59,13 +66,15 @@ namespace pf {
 //   ms against 2.49 with the row-interleaved coefficients)
 //   (timing copy with the coefficient staging left out: 1.93 ms; with the
 //   scatter left out 2.73, with the normals and curvature left out 2.68)
-// z-chunks of 16, 32 and 64 planes are within 2 %.
+// z-chunks of 16, 32 and 64 planes are within 2 %.  The droplet's step
+// (periodic in z) spills registers at four workgroups per CU; it runs on
+// small grids only.
 constexpr int kMomTY = kMomRows;
 #ifndef PF_MOM_PREFETCH
 #define PF_MOM_PREFETCH 0
 #endif
 #ifndef PF_MOM_WAVES
-#define PF_MOM_WAVES 3
+#define PF_MOM_WAVES 4
 #endif
 // PF_MOM_FAST: the hardware reciprocal and square root (1 ulp) for |grad
 // phi|, the normals, omega and 1/rho (recip / root of step_kernels.hpp);
@@ -103,13 +112,14 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
     constexpr int kThreads = TX * TY;
     constexpr int HX = TX + 4, HY = TY + 4;    // phi tile with 2-cell halo
     constexpr int NX1 = TX + 2, NY1 = TY + 2;  // normal and coefficient tiles with 1-cell halo
-    // LDS: a 4-plane phi ring, 3 planes of n with a 1-cell halo, |grad phi|
-    // of 2 planes at the tile cells (it is read at the centre only), and the
+    // LDS (40.3 KB at 64 x 4, so that four workgroups share a CU): a 3-plane
+    // phi ring, the normals of one plane with a 1-cell halo, |grad phi| of 2
+    // planes at the tile cells (it is read at the centre only), and the
     // coefficients of one sending plane with a 1-cell halo
-    constexpr int kRing = 4;
+    constexpr int kRing = 3;
     constexpr int kRec = NX1 * NY1;
     __shared__ float s_phi[kRing][HY * HX];
-    __shared__ float s_n[3][3][NY1 * NX1];
+    __shared__ float s_n[3][NY1 * NX1];
     __shared__ float s_ind[2][kThreads];
     __shared__ float s_rec[NC][kRec];
 
@@ -185,7 +195,6 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
     // normals and |grad phi| on the 1-halo tile of plane pz
     // (src/phaseField.cuh:54-140), as k_step
     auto comp_normals = [&](int pz) {
-        const int r = pz % 3;
         const float* pm = s_phi[(pz + kRing - 1) % kRing];
         const float* p0 = s_phi[pz % kRing];
         const float* pp = s_phi[(pz + 1) % kRing];
@@ -210,11 +219,46 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
                 nry = gy_ * inv;
                 nrz = gz_ * inv;
             }
-            s_n[r][0][i] = nrx;
-            s_n[r][1][i] = nry;
-            s_n[r][2][i] = nrz;
+            s_n[0][i] = nrx;
+            s_n[1][i] = nry;
+            s_n[2][i] = nrz;
             if (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY) s_ind[pz & 1][(ly - 1) * TX + (lx - 1)] = ind;
         }
+    };
+
+    // Divergence of n (src/phaseField.cuh:168-213) marched in z as the
+    // moments are: from the normals of plane pw in s_n, the terms
+    // w_q c_q . n(x + c_q) of this thread's column whose neighbour lies in
+    // pw, summed per c_z: tm (c_z = -1) for the cell at pw + 1, t0 for the
+    // cell at pw, tp (c_z = +1) for the cell at pw - 1.  Every cell adds
+    // those of planes z - 1, z, z + 1 in that order.  nself, phself: n and
+    // phi of the column's own cell in pw.
+    const int o = (ty + 1) * NX1 + (tx + 1);
+    auto div_terms = [&](int pw, float& tm, float& t0, float& tp, float (&nself)[3], float& phself) {
+        phself = s_phi[pw % kRing][(ty + 2) * HX + (tx + 2)];
+        float s0 = 0.0f, s1 = 0.0f, b1 = 0.0f;
+        static_for<9>([&](auto oc) {
+            constexpr int oo = decltype(oc)::value, cx = oo % 3 - 1, cy = oo / 3 - 1;
+            const int src = o + cy * NX1 + cx;
+            const float nz_ = s_n[2][src];
+            float sxy = 0.0f;
+            if constexpr (cx != 0) sxy = (float)cx * s_n[0][src];
+            if constexpr (cy != 0) sxy = sxy + (float)cy * s_n[1][src];
+            if constexpr (cx == 0 && cy == 0) {
+                nself[0] = s_n[0][src];
+                nself[1] = s_n[1][src];
+                nself[2] = nz_;
+            }
+            if constexpr (cx != 0 || cy != 0) s0 = s0 + wk<Q>((cx != 0) + (cy != 0)) * sxy;
+            if constexpr (mom_has<Q>(cx, cy, 1)) {
+                constexpr float w1 = wk<Q>((cx != 0) + (cy != 0) + 1);
+                if constexpr (cx != 0 || cy != 0) s1 = s1 + w1 * sxy;
+                b1 = b1 + w1 * nz_;
+            }
+        });
+        t0 = s0;
+        tp = s1 + b1;
+        tm = s1 - b1;
     };
 
     // Rebuild the populations sent by plane pw to this thread's column and
@@ -269,6 +313,10 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
 
     // prologue: phi planes zs-2 .. zs+1, normals of zs-1 and zs, and the
     // arrivals of plane zs from zs-1 and zs (and of zs+1 from zs)
+    MomAcc a_lo = mom_zero(), a_mid = mom_zero(), a_hi = mom_zero(), a_off = mom_zero();
+    // curvature sums of the cells at z (k_lo) and z + 1 (k_mid); n and phi
+    // of the cell at z (n_cur, ph_cur)
+    float k_lo, k_mid, k_off, n_cur[3], n_next[3], ph_cur, ph_next;
     {
         float v[4][kPhiIt];
 #pragma unroll
@@ -276,21 +324,26 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
         float rv[kRIt][NC];
         if constexpr (!LOCAL) fetch_rec(zs - 1, rv);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) put_phi(zs - 2 + k, v[k]);
+        for (int k = 0; k < 3; ++k) put_phi(zs - 2 + k, v[k]);
         if constexpr (!LOCAL) put_rec(rv);
-    }
-    lds_barrier();
-    comp_normals(zs - 1);
-    comp_normals(zs);
-    MomAcc a_lo = mom_zero(), a_mid = mom_zero(), a_hi = mom_zero(), a_off = mom_zero();
-    if constexpr (!LOCAL) {
-        scatter(F{}, F{}, T{}, zs - 1, a_off, a_off, a_lo);
         lds_barrier();
-        float rv[kRIt][NC];
-        fetch_rec(zs, rv);
-        put_rec(rv);
+        comp_normals(zs - 1);
+        if constexpr (!LOCAL) scatter(F{}, F{}, T{}, zs - 1, a_off, a_off, a_lo);
         lds_barrier();
-        scatter(F{}, T{}, T{}, zs, a_off, a_lo, a_mid);
+        div_terms(zs - 1, k_lo, k_off, k_off, n_next, ph_next);
+        if constexpr (!LOCAL) fetch_rec(zs, rv);
+        lds_barrier();
+        put_phi(zs + 1, v[3]);  // into the slot of zs - 2
+        if constexpr (!LOCAL) put_rec(rv);
+        lds_barrier();
+        comp_normals(zs);
+        lds_barrier();
+        if constexpr (!LOCAL) scatter(F{}, T{}, T{}, zs, a_off, a_lo, a_mid);
+        {
+            float t0;
+            div_terms(zs, k_mid, t0, k_off, n_cur, ph_cur);
+            k_lo = k_lo + t0;
+        }
     }
     lds_barrier();
 
@@ -323,29 +376,32 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_st
             fetch_phi(z + 3, phv);
         }
 
-        // the arrivals of plane z are complete
+        // the arrivals of plane z are complete, and with the normals of
+        // plane z + 1 its curvature sum
         MomAcc a = a_lo;
         a_lo = a_mid;
         a_mid = a_hi;
         a_hi = mom_zero();
+        float ksum;
+        {
+            float tm, t0, tp;
+            div_terms(z + 1, tm, t0, tp, n_next, ph_next);
+            ksum = k_lo + tp;
+            k_lo = k_mid + t0;
+            k_mid = tm;
+        }
+        const float nrx = n_cur[0], nry = n_cur[1], nrz = n_cur[2];
+        n_cur[0] = n_next[0];
+        n_cur[1] = n_next[1];
+        n_cur[2] = n_next[2];
+        const float phi = ph_cur;
+        ph_cur = ph_next;
 
         // ---- curvature and continuum surface force (src/phaseField.cuh:142-219)
-        const int o = (ty + 1) * NX1 + (tx + 1);
-        const float* nm[3] = {s_n[(z + 2) % 3][0], s_n[z % 3][0], s_n[(z + 1) % 3][0]};
-        auto nval = [&](int comp_, auto qc) {
-            constexpr int q = decltype(qc)::value;
-            constexpr int off = kCY[q] * NX1 + kCX[q];
-            return nm[kCZ[q] + 1][comp_ * (NY1 * NX1) + o + off];
-        };
-        const float phi = s_phi[z % kRing][(ty + 2) * HX + (tx + 2)];
         if (!active) continue;
-        const float scx = iso_sum<Q, 0>([&](auto qc) { return nval(0, qc); });
-        const float scy = iso_sum<Q, 1>([&](auto qc) { return nval(1, qc); });
-        const float scz = iso_sum<Q, 2>([&](auto qc) { return nval(2, qc); });
-        const float curvature = kAs2 * (scx + scy + scz);
+        const float curvature = kAs2 * ksum;
         const uint32_t c = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z);
         const float ind = s_ind[z & 1][tid];
-        const float nrx = s_n[z % 3][0][o], nry = s_n[z % 3][1][o], nrz = s_n[z % 3][2][o];
         const float stCurv = -p.sigma * curvature * ind;
         const float fsx = stCurv * nrx, fsy = stCurv * nry, fsz = stCurv * nrz;
 

@@ -4,17 +4,20 @@
 // with the populations kept as the coefficients of their senders' collision
 // output instead of stored one by one.  A workgroup marches a TX x TY column
 // of cells through its z-chunk.  Per plane:
-//   - a phi ring and the normals of plane w, as in k_step;
+//   - a phi ring and the normals of plane w, as in k_step; each thread adds
+//     their divergence terms to the curvature sums of its column's cells in
+//     planes w - 1, w and w + 1;
 //   - the coefficients of one sending plane w, with a 1-cell halo, are staged
 //     in LDS, and each thread rebuilds from them the populations its column
 //     receives from plane w: those with c_z = +1 belong to its cell in plane
 //     w + 1, c_z = 0 to plane w, c_z = -1 to plane w - 1.  It sums each c_z
 //     layer into the raw moments (sum f, sum f c, sum f c c) of the cell it
 //     arrives at, so it carries the partial moments of three cells;
-//   - the cell of plane w - 1 then has all its arrivals (from planes w - 2,
-//     w - 1, w: always in that order, so the sums do not depend on the tile
-//     or the z-chunk): its collision writes its own coefficients, the phase
-//     flux, and on save steps the macroscopic fields.
+//   - the cell of plane w - 1 then has all its arrivals and curvature terms
+//     (from planes w - 2, w - 1, w: always in that order, so the sums do not
+//     depend on the tile or the z-chunk): its collision writes its own
+//     coefficients, the phase flux, and on save steps the macroscopic
+//     fields.
 // The jet's ghost planes send the boundary populations of k_jet_bc (p.bcf)
 // instead.  The first step after initialisation (local) sums each cell's
 // own coefficients, the equilibrium setDistros leaves in every slot

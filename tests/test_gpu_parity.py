@@ -30,7 +30,7 @@ import pytest
 
 import phasefieldlbm_amd as pf
 from phasefieldlbm_amd import (D3Q19, D3Q27, NOISE_GAUSSIAN_XY, NOISE_NONE, NOISE_UNIFORM_Z, STORE_FP16, STORE_FP16_NE,
-                               presets)
+                               STORE_FP32, STORE_MOM, presets)
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -624,3 +624,39 @@ def test_graph_replay_matches_direct_launches(case):
         b.init().step(5, graph=True).step(1, save=True, graph=True)
         for k in CORE:
             np.testing.assert_array_equal(a.field(k), b.field(k), err_msg=f"{case} after re-init {k}")
+
+
+@pytest.mark.parametrize("storage", [STORE_FP32, STORE_MOM])
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+def test_caller_fields_then_init_distros(Q, storage):
+    """pf_set_field then pf_init_distros (the reference's setDistros,
+    src/initialConditions.cu:101-138, on a caller's state): an ellipsoidal
+    drop with density noise moving diagonally, 30 steps, against the oracle
+    given the same fields, to 1e-5 relative L-infinity."""
+    cfg = presets.static_droplet(n=24, radius=6.0, velocity_set=Q, storage=storage)
+    z, y, x = np.meshgrid(*(np.arange(n, dtype=np.float64) for n in cfg.shape), indexing="ij")
+    cx, cy, cz = ((n - 1) / 2 for n in (cfg.nx, cfg.ny, cfg.nz))
+    r = np.sqrt(((x - cx) / 1.3) ** 2 + (y - cy) ** 2 + ((z - cz) / 0.8) ** 2)
+    phi = 0.5 * (1.0 + np.tanh(2.0 * (6.0 - r) / 4.0))
+    rng = np.random.default_rng(5)
+    fields = {"phi": phi, "rho": 1.0 + 1e-3 * rng.standard_normal(cfg.shape),
+              "ux": 0.02 * phi, "uy": -0.01 * phi, "uz": 0.015 * phi}
+    fields = {k: v.astype(np.float32) for k, v in fields.items()}
+    print(f"\n[caller fields] Q={Q} storage={storage}", flush=True)
+    with pf.Simulation(cfg) as sim:
+        sim.init()
+        for k, v in fields.items():
+            sim.set_field(k, v)
+        sim.init_distros()
+        sim.step(30, save=True)
+        got = {k: sim.field(k) for k in CORE}
+    o = oracle.Oracle(cfg, 32).init()
+    for k, v in fields.items():
+        o.set_field(k, v)
+    o.init_distros()
+    o.step(30)
+    errs = {k: rel_linf(got[k][I], o.field(k)[I]) for k in CORE}
+    print("  errors", errs, flush=True)
+    for k, e in errs.items():
+        assert e < 1e-5, (k, e)
+

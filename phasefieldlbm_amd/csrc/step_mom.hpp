@@ -37,7 +37,9 @@ namespace pf {
 //                                                    2.37 ms
 //   the same with 32-plane z-chunks 2.40-2.42 ms, 24-plane 2.39 ms, at 3
 //   workgroups per CU (134 registers) 2.42-2.45 ms; tile rows dealt
-//   round-robin (PF_MOM_BANDS=2) 2.37 ms, the hardware order 3.17 ms
+//   round-robin (PF_MOM_BANDS=2) 2.37 ms, the hardware order 3.17 ms;
+//   every eighth z-chunk layer to one XCD (all tile rows of a layer on one
+//   XCD) 2.61-2.66 ms
 //   64 x 8 (66.7 KB, 2 workgroups of 8 waves per CU, a 1.29x halo tile
 //   against 1.55x): 2.50-2.53 ms
 //   (timing copies, results discarded: without the coefficient staging

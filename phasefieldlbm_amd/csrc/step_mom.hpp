@@ -42,7 +42,8 @@ namespace pf {
 //   XCD) 2.61-2.66 ms
 //   64 x 8 (66.7 KB, 2 workgroups of 8 waves per CU, a 1.29x halo tile
 //   against 1.55x): 2.50-2.53 ms; 128 x 4 (79.2 KB, 2 of 8 waves per CU at
-//   a 128-register cap): 2.57-2.62 ms
+//   a 128-register cap): 2.57-2.62 ms; 32 x 4 (20.8 KB, 7 workgroups of 2
+//   waves per CU): 2.69 ms
 //   (timing copies, results discarded: without the coefficient staging
 //   1.60 ms, without the scatter 2.08, without the normals 2.08 against
 //   2.34; the fetched traffic runs at 0.91 of the copy rate)

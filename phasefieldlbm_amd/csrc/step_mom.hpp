@@ -35,7 +35,8 @@ namespace pf {
 //   64 x 4, 4 workgroups per CU, the curvature marched in z (one plane of
 //   normals in LDS) and a 3-plane phi ring, 16-plane z-chunks
 //                                                    2.37 ms
-//   the same with 32-plane z-chunks 2.40-2.42 ms, 24-plane 2.39 ms, at 3
+//   the same with 24-plane z-chunks 2.39 ms, 32-plane 2.40-2.42, 64-plane
+//   2.41, 128-plane 2.46, 256-plane (one round of 1024 workgroups) 2.61; at 3
 //   workgroups per CU (134 registers) 2.42-2.45 ms; tile rows dealt
 //   round-robin (PF_MOM_BANDS=2) 2.37 ms, the hardware order 3.17 ms;
 //   every eighth z-chunk layer to one XCD (all tile rows of a layer on one

@@ -47,7 +47,7 @@ def jet_consts(nz, diam, nx=8, ny=8, re_oil=5000.0, prec=64):
     return cfg, o, o.consts()
 
 
-# The 128x128x512 and 256x256x1024 jets, the reference's
+# The 128x128x512 (D = 16) and 256x256x1024 (D = 32) jets at Re = 5000, We = 500, the reference's
 # own 64x64x128 jet, and nz = 120, where nz/12 and (nz-1)/12 differ.
 CASES = [(512, 16, 1.6e-4, 0.50048, 8e-5, 42), (1024, 32, 3.2e-4, 0.50096, 1.6e-4, 85),
          (128, 10, 1.0e-4, 0.5003, 5e-5, 10), (120, 12, 1.2e-4, 0.50036, 6e-5, 10)]

@@ -46,13 +46,24 @@ def test_mom_jet(Q, noise):
     compare(cfg, 60, fields=pf.FIELDS, region=(slice(None),) * 3)
 
 
-def test_mom_reference_jet_gaussian():
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+def test_mom_reference_jet_gaussian(Q):
     """The reference's default jet (64x64x128, D = 10, Gaussian inflow noise
     through logf/cosf, whose last-ulp behaviour differs between the GPU and
     the host C library): 1e-4."""
-    cfg = presets.reference_jet(D3Q27)
+    cfg = presets.reference_jet(Q)
     cfg.storage = STORE_MOM
     compare(cfg, 40, tol=1e-4)
+
+
+@pytest.mark.parametrize("Q", [D3Q19, D3Q27])
+def test_mom_jet_partial_tiles(Q):
+    """A 70 x 30 cross-section: a full and a partial 64-wide tile in x,
+    partial 4-row tiles in y, and 16-plane chunks with a partial last one;
+    every field, ghost planes included, after 30 steps."""
+    cfg = presets.round_jet(70, 30, 37, 8, Q, storage=STORE_MOM, noise=NOISE_UNIFORM_Z)
+    cfg.zchunk = 16
+    compare(cfg, 30, fields=pf.FIELDS, region=(slice(None),) * 3)
 
 
 @pytest.mark.parametrize("steps", [0, 1, 4, 5])

@@ -81,9 +81,10 @@ namespace pf {
 //   (timing copy with the coefficient staging left out: 1.93 ms; with the
 //   scatter left out 2.73, with the normals and curvature left out 2.68)
 // z-chunks of 16, 32 and 64 planes are within 2 %.  The droplet's step
-// (periodic in z) keeps a 168-register cap, 3 waves per SIMD: at 128
-// registers its spills made it slower (D3Q27 128^3 droplet: 0.170 against
-// 0.196 ms).
+// (periodic in z) guards its scatter with a branch as the jet's boundary
+// planes do; without it the compiler hoisted all nine columns' LDS reads
+// together and spilled (D3Q27 128^3 droplet: 0.104 ms against 0.170 at a
+// 168-register cap and 0.196 at 128).
 constexpr int kMomTY = kMomRows;
 #ifndef PF_MOM_PREFETCH
 #define PF_MOM_PREFETCH 0
@@ -120,7 +121,7 @@ __device__ __forceinline__ uint32_t xcd_band_remap(uint32_t b, uint32_t nblk, ui
 }
 
 template <int Q, bool JET, bool SAVE, bool LOCAL, int TX>
-__global__ __launch_bounds__(TX * kMomTY, TX == 64 ? (JET ? PF_MOM_WAVES : 3) : 2) void k_step_mom(KParams p) {
+__global__ __launch_bounds__(TX * kMomTY, TX == 64 ? PF_MOM_WAVES : 2) void k_step_mom(KParams p) {
     constexpr int NC = mom_nc<Q>();
     constexpr bool kFast = PF_MOM_FAST != 0;
     constexpr int TY = kMomTY;
@@ -306,6 +307,12 @@ __global__ __launch_bounds__(TX * kMomTY, TX == 64 ? (JET ? PF_MOM_WAVES : 3) : 
                 if constexpr (DM) layer_to<-1>(am, bc_layer(9));
                 return;
             }
+        }
+        if constexpr (!JET) {
+            // a plane outside the grid sends nothing (the chunking never asks
+            // for one; as a branch it also keeps the compiler from hoisting
+            // all nine columns' reads together, which spilled registers)
+            if (pw < 0 || pw > nz - 1) return;
         }
         MomLayer gm{0, 0, 0, 0, 0, 0, false}, g0 = gm, gp = gm;
         static_for<9>([&](auto oc) {

@@ -183,21 +183,25 @@ int choose_zchunk(const pf_config& g) {
     if (g.zchunk > 0) return g.zchunk;
     const int tw = tile_x(g), th = tile_rows(g, tw);
     const long cols = (long)((g.nx + tw - 1) / tw) * ((g.ny + th - 1) / th);
-    // the binary16 and moment forms march long chunks
-    const bool fp16 = g.storage == PF_STORE_FP16 || g.storage == PF_STORE_MOM;
-    const long target = fp16 ? 8192 : 32768;
-    long zc = ((long)(g.nz - 2) * cols + target / 2) / target;
-    if (zc < 4) zc = 4;
-    if (zc > 64) zc = 64;
-    // The moment form stages two sending planes more than it updates per
-    // chunk: 16 planes where the grid fills the GPU with them.  Its step took
-    // 2.37 ms at 16 planes and 2.40-2.42 at 32 on the 256x256x1024 D3Q27
-    // jet, and 0.220 ms at 8 or 16 planes against 0.232 at 4 on the
-    // 128x128x512 D3Q19 jet.
-    if (g.storage == PF_STORE_MOM) zc = 16;
-    // FP32 grids that reach the target with 4-plane chunks take half the
-    // chunk, at least 2 planes (see the measurements above).
-    if (!fp16 && ((long)(g.nz - 2) + 3) / 4 * cols >= target) zc = std::max(2L, zc / 2);
+    long zc;
+    if (g.storage == PF_STORE_MOM) {
+        // The moment form stages two sending planes more than it updates per
+        // chunk: 16 planes where the grid fills the GPU with them.  Its step
+        // took 2.37 ms at 16 planes and 2.40-2.42 at 32 on the 256x256x1024
+        // D3Q27 jet, and 0.220 ms at 8 or 16 planes against 0.232 at 4 on
+        // the 128x128x512 D3Q19 jet.
+        zc = 16;
+    } else {
+        // binary16 marches long chunks
+        const bool fp16 = g.storage == PF_STORE_FP16;
+        const long target = fp16 ? 8192 : 32768;
+        zc = ((long)(g.nz - 2) * cols + target / 2) / target;
+        if (zc < 4) zc = 4;
+        if (zc > 64) zc = 64;
+        // FP32 grids that reach the target with 4-plane chunks take half the
+        // chunk, at least 2 planes (see the measurements above).
+        if (!fp16 && ((long)(g.nz - 2) + 3) / 4 * cols >= target) zc = std::max(2L, zc / 2);
+    }
     constexpr long kCUs = 256;
     while (zc > 1 && ((long)(g.nz - 2) + zc - 1) / zc * cols < kCUs) --zc;
     return (int)zc;
